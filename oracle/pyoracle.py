@@ -1,0 +1,158 @@
+"""ctypes binding of the CPU restatement in chyoracle.c (TEST INFRASTRUCTURE).
+
+Only tests/, the smoke() entry and bench.py's cpu_baseline leg use this module,
+and only as the checker.  The library is built by ``make -C oracle`` (also run
+by ``__graft_entry__.build()``) into ``oracle/_build/libchyoracle.so``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "libchyoracle.so"
+EXT_LIB_PATH = HERE / "_build" / "libchyoracle_ext.so"
+_libs = {}
+_extended = False
+
+
+def use_extended(flag: bool):
+    """Route the calls below to the 80-bit build (same algorithm, same FP64 inputs)."""
+    global _extended
+    _extended = bool(flag)
+
+
+def lib():
+    path = EXT_LIB_PATH if _extended else LIB_PATH
+    if path not in _libs:
+        if not path.exists():
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        L = C.CDLL(str(path))
+        d = C.POINTER(C.c_double)
+        i = C.POINTER(C.c_int)
+        L.ora_rfft_fwd.argtypes = [C.c_int, d, d, d]
+        L.ora_rfft_inv.argtypes = [C.c_int, d, d, d]
+        L.ora_mode_solve.argtypes = [C.c_int, C.c_int, d, d, d, d, d, d, d]
+        grid = [C.c_int, C.c_int, C.c_int, d, d, d, d, C.c_double, C.c_double, C.c_double]
+        L.ora_pot3.argtypes = grid + [C.c_int, d, d, d, d, i]
+        L.ora_bdygen.argtypes = grid + [C.c_int, C.c_int, C.c_double, d, d, d]
+        L.ora_residual.argtypes = [C.c_int, C.c_int, C.c_int, d, d, C.c_double, C.c_double,
+                                   C.c_double, d, d, d, d]
+        L.ora_terms.argtypes = [i, i, d, i, d]
+        _libs[path] = L
+    return _libs[path]
+
+
+def terms():
+    """The (l, m) multipole table: list of (l, m, dlm, poly[np]) with l+m even."""
+    l = (C.c_int * 64)(); m = (C.c_int * 64)(); np_ = (C.c_int * 64)()
+    dlm = np.zeros(64); poly = np.zeros(64 * 6)
+    n = lib().ora_terms(l, m, _p(dlm), np_, _p(poly))
+    return [(l[t], m[t], float(dlm[t]), poly[t * 6:t * 6 + np_[t]].copy()) for t in range(n)]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def rfft_fwd(x):
+    """Raw FFT(+1)+REALTR(+1) of a real lane: returns (A[0..N], B[0..N])."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    L = x.size
+    A = np.zeros(L // 2 + 1)
+    B = np.zeros(L // 2 + 1)
+    rc = lib().ora_rfft_fwd(L, _p(x), _p(A), _p(B))
+    if rc:
+        raise ValueError(f"ora_rfft_fwd rc={rc}")
+    return A, B
+
+
+def rfft_inv(A, B):
+    """Raw REALTR(-1)+FFT(-1) with B[0]=B[N]=0: returns the L interleaved samples."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    L = 2 * (A.size - 1)
+    out = np.zeros(L)
+    rc = lib().ora_rfft_inv(L, _p(A), _p(B), _p(out))
+    if rc:
+        raise ValueError(f"ora_rfft_inv rc={rc}")
+    return out
+
+
+def mode_solve(an, bn, cn, am, bm, cm, y):
+    """Solve the BLKTRI system; y is Y(M,N) as a numpy (N,M) row-major array."""
+    y = np.array(y, dtype=np.float64, order="C", copy=True)
+    n, m = y.shape
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (an, bn, cn, am, bm, cm)]
+    rc = lib().ora_mode_solve(m, n, *[_p(a) for a in args], _p(y))
+    if rc:
+        raise ValueError(f"ora_mode_solve rc={rc}")
+    return y
+
+
+def _grid_args(g):
+    r = np.ascontiguousarray(g.r())
+    z = np.ascontiguousarray(g.z())
+    rhf = np.ascontiguousarray(g.rhf())
+    zhf = np.ascontiguousarray(g.zhf())
+    keep = (r, z, rhf, zhf)
+    return keep, [g.jmax, g.kmax, g.lmax, _p(r), _p(z), _p(rhf), _p(zhf), g.dtheta,
+                  3.1415926535897932, 1.0]
+
+
+def bdygen(g, rho, phi=None, maxtrm=10, isym=2, redge=0.0):
+    """BDYGEN: returns (phi with boundary values, tmass)."""
+    rho = np.asfortranarray(rho, dtype=np.float64)
+    phi = np.zeros(g.shape, order="F") if phi is None else np.array(phi, order="F", copy=True)
+    tm = np.zeros(1)
+    keep, ga = _grid_args(g)
+    rc = lib().ora_bdygen(*ga, maxtrm, isym, redge, _p(rho), _p(phi), _p(tm))
+    if rc:
+        raise ValueError(f"ora_bdygen rc={rc}")
+    return phi, float(tm[0])
+
+
+def pot3(g, rho, phi, npoint=8, coef=None, klocat=0):
+    """POT3 on phi (holding boundary values): returns (phi, coef, phichk, klocat)."""
+    rho = np.asfortranarray(rho, dtype=np.float64)
+    phi = np.array(phi, order="F", copy=True, dtype=np.float64)
+    if coef is None:
+        coef = np.zeros((g.jmax + 2, g.kmax + 2, g.lmax // 2, 2), order="F")
+    else:
+        coef = np.array(coef, order="F", copy=True, dtype=np.float64)
+    pc = np.zeros(1)
+    kl = C.c_int(klocat)
+    keep, ga = _grid_args(g)
+    rc = lib().ora_pot3(*ga, npoint, _p(rho), _p(phi), _p(coef), _p(pc), C.byref(kl))
+    if rc:
+        raise ValueError(f"ora_pot3 rc={rc}")
+    return phi, coef, float(pc[0]), int(kl.value)
+
+
+def solve(g, rho, maxtrm=10, isym=2, redge=0.0, npoint=8):
+    """SETBDY+BDYGEN+POT3 on a zero-initialised PHI, as the ITYPE=71 driver."""
+    phi, tmass = bdygen(g, rho, None, maxtrm, isym, redge)
+    phib = phi.copy()
+    phi, coef, phichk, klocat = pot3(g, rho, phi, npoint)
+    return dict(phi_b=phib, tmass=tmass, phi=phi, coef=coef, phichk=phichk, klocat=klocat)
+
+
+def residual(g, phi, rho):
+    """Residual A phi - 4 pi G rho of the discrete operator on J=2..JMAX, K=2..KMAX,
+    and the per-cell sum of |terms| (the scale of its rounding floor)."""
+    phi = np.asfortranarray(phi, dtype=np.float64)
+    rho = np.asfortranarray(rho, dtype=np.float64)
+    res = np.zeros(g.shape, order="F")
+    scale = np.zeros(g.shape, order="F")
+    rhf = np.ascontiguousarray(g.rhf())
+    zhf = np.ascontiguousarray(g.zhf())
+    lib().ora_residual(g.jmax, g.kmax, g.lmax, _p(rhf), _p(zhf), g.dtheta, 3.1415926535897932, 1.0,
+                       _p(phi), _p(rho), _p(res), _p(scale))
+    return res, scale
+
+
+def set_threads(n: int):
+    os.environ["OMP_NUM_THREADS"] = str(n)

@@ -1,0 +1,299 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Runs only where the reference tree exists (it builds oracle/_ref/ through
+oracle/ref/Makefile, i.e. the reference's own pot/pot3.f, fft.f, blktri.f and
+boundary.f compiled with flang -fdefault-real-8 -fdefault-double-8) and stores
+inputs and outputs as small .npz files.  The reference's OpenMP code gives
+thread-count-dependent results under this flang, so every run here uses
+OMP_NUM_THREADS=1.
+
+    python tests/golden/make_golden.py            # all fixtures
+    python tests/golden/make_golden.py --only fft # one family
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import pathlib
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT / "chymera-uo_amd"))
+from chypot import fields  # noqa: E402
+
+GOLDEN = ROOT / "tests" / "golden"
+REFBIN = ROOT / "oracle" / "_ref"
+
+
+def build(grids):
+    subprocess.run(["make", "-C", str(ROOT / "oracle" / "ref"), f"GRIDS={' '.join(grids)}",
+                    "VARIANTS=r8", "-j4"], check=True, stdout=subprocess.DEVNULL)
+
+
+def env1():
+    e = dict(os.environ)
+    e["OMP_NUM_THREADS"] = "1"
+    e["OMP_STACKSIZE"] = "512M"
+    return e
+
+
+def run_solve(g: fields.Grid, rho: np.ndarray, isym=2, maxtrm=10, npoint=8, redge=0.0,
+              nwarm=0, nrep=0, binary=None):
+    exe = binary or (REFBIN / f"r8_{g.name}" / "refpot")
+    with tempfile.TemporaryDirectory() as td:
+        fin = pathlib.Path(td) / "in.bin"
+        fout = pathlib.Path(td) / "out.bin"
+        with open(fin, "wb") as f:
+            f.write(np.array([g.jmax, g.kmax, g.lmax, isym, maxtrm, npoint, nwarm, nrep],
+                             dtype="<i4").tobytes())
+            f.write(np.array([g.delr, g.delz, redge, 0.0], dtype="<f8").tobytes())
+            f.write(np.asarray(rho, dtype="<f8").tobytes(order="F"))
+        subprocess.run([str(exe), "solve", str(fin), str(fout)], check=True, env=env1())
+        raw = fout.read_bytes()
+    n = int(np.prod(g.shape))
+    nc = (g.jmax + 2) * (g.kmax + 2) * (g.lmax // 2) * 2
+    d = np.frombuffer(raw, dtype="<f8", count=2 * n + 1 + nc + 3)
+    out = {
+        "phi_b": d[:n].reshape(g.shape, order="F").copy(),
+        "tmass_b": float(d[n]),
+        "phi": d[n + 1:2 * n + 1].reshape(g.shape, order="F").copy(),
+        "coef": d[2 * n + 1:2 * n + 1 + nc].reshape((g.jmax + 2, g.kmax + 2, g.lmax // 2, 2),
+                                                   order="F").copy(),
+    }
+    tmass, phichk, klocat = d[2 * n + 1 + nc:]
+    out.update(tmass=float(tmass), phichk=float(phichk), klocat=int(klocat))
+    if nrep > 0:
+        off = (2 * n + 1 + nc + 3) * 8
+        out["threads"] = int(np.frombuffer(raw, dtype="<i4", count=1, offset=off)[0])
+        out["seconds"] = np.frombuffer(raw, dtype="<f8", count=nrep, offset=off + 4).copy()
+    return out
+
+
+def rho_digest(rho: np.ndarray) -> str:
+    return hashlib.sha256(np.asarray(rho, dtype="<f8").tobytes(order="F")).hexdigest()
+
+
+# ---------------------------------------------------------------------------
+# FFT + REALTR: forward and inverse on random vectors (pot/fft.f)
+
+def make_fft():
+    g = fields.unit_grid(16, 8, 16)
+    build([g.name])
+    exe = REFBIN / f"r8_{g.name}" / "refpot"
+    rng = np.random.default_rng(11)
+    out = {}
+    for n in (1, 2, 3, 4, 5, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 256):
+        nvec = 4
+        x = rng.uniform(-1.0, 1.0, size=(nvec, 2 * n))
+        with tempfile.TemporaryDirectory() as td:
+            fin = pathlib.Path(td) / "in.bin"
+            fout = pathlib.Path(td) / "out.bin"
+            with open(fin, "wb") as f:
+                f.write(np.array([n, nvec], dtype="<i4").tobytes())
+                f.write(x.astype("<f8").tobytes())  # row-major (nvec,2n) == Fortran x(2n,nvec)
+            subprocess.run([str(exe), "fft", str(fin), str(fout)], check=True, env=env1())
+            d = np.fromfile(fout, dtype="<f8").reshape(nvec, -1)
+        a = d[:, :n + 1]
+        b = d[:, n + 1:2 * n + 2]
+        ia = d[:, 2 * n + 2:3 * n + 2]
+        ib = d[:, 3 * n + 2:4 * n + 2]
+        out[f"x_{n}"] = x
+        out[f"a_{n}"] = a
+        out[f"b_{n}"] = b
+        out[f"ia_{n}"] = ia
+        out[f"ib_{n}"] = ib
+    np.savez_compressed(GOLDEN / "fft_realtr.npz", **out)
+
+
+# ---------------------------------------------------------------------------
+# BLKTRI on POT3-shaped systems (pot/blktri.f), coefficients as pot3.f:158-231
+
+def pot3_coeffs(g: fields.Grid, m: int):
+    """Per-mode coefficient arrays in the form POT3 hands them to BLKTRI."""
+    rhf, zhf = g.rhf(), g.zhf()
+    J, K = g.jmax, g.kmax
+    rd2 = np.zeros(J + 1); rd3 = np.zeros(J + 1); denr = np.zeros(J + 1)
+    rd2[1] = 0.5 / rhf[1]
+    rd3[1] = 1.0 / rhf[0]
+    for j in range(2, J + 1):  # Fortran J
+        denr[j] = 1.0 / (rhf[j] - rhf[j - 2])
+        rd2[j] = 1.0 / (rhf[j] - rhf[j - 1])
+        rd3[j] = 1.0 / rhf[j - 1]
+    zd2 = np.zeros(K + 1); denz = np.zeros(K + 1)
+    zd2[1] = 0.5 / zhf[1]
+    for k in range(2, K + 1):
+        denz[k] = 1.0 / (zhf[k] - zhf[k - 2])
+        zd2[k] = 1.0 / (zhf[k] - zhf[k - 1])
+    cm = np.array([2.0 * (rd2[j] + 0.5 * rd3[j]) * denr[j] for j in range(2, J + 1)])
+    am = np.array([2.0 * (rd2[j - 1] - 0.5 * rd3[j]) * denr[j] for j in range(2, J + 1)])
+    c = np.array([rd3[j] * rd3[j] / (g.dtheta * g.dtheta) for j in range(2, J + 1)])
+    cn = np.array([2.0 * zd2[k] * denz[k] for k in range(2, K + 1)])
+    an = np.array([2.0 * zd2[k - 1] * denz[k] for k in range(2, K + 1)])
+    bn = -cn - an
+    cmmax = cm[-1]; cm[-1] = 0.0
+    ammin = am[0]; am[0] = 0.0
+    cn[-1] = 0.0
+    bn[0] = -cn[0]; an[0] = 0.0
+    lam = np.cos(m * g.dtheta)
+    bm = -cm - am + 2.0 * (lam - 1.0) * c
+    bm[0] = -cm[0] + ((-1.0) ** m - 1.0) * ammin + 2.0 * (lam - 1.0) * c[0]
+    bm[-1] = -cmmax - am[-1] + 2.0 * (lam - 1.0) * c[-1]
+    return an, bn, cn, am, bm, cm
+
+
+def make_blktri():
+    g0 = fields.unit_grid(16, 8, 16)
+    build([g0.name])
+    exe = REFBIN / f"r8_{g0.name}" / "refpot"
+    rng = np.random.default_rng(12)
+    out = {}
+    cases = [(fields.unit_grid(8, 4, 16), 0), (fields.unit_grid(8, 4, 16), 3),
+             (fields.unit_grid(16, 8, 16), 1), (fields.unit_grid(24, 16, 32), 5),
+             (fields.unit_grid(33, 32, 32), 16), (fields.Grid(20, 16, 64, 0.03, 0.011), 2)]
+    for i, (g, m) in enumerate(cases):
+        an, bn, cn, am, bm, cm = pot3_coeffs(g, m)
+        M, N = g.jmax - 1, g.kmax - 1
+        y = rng.uniform(-1.0, 1.0, size=(N, M))  # row-major (N,M) == Fortran y(M,N)
+        with tempfile.TemporaryDirectory() as td:
+            fin = pathlib.Path(td) / "in.bin"
+            fout = pathlib.Path(td) / "out.bin"
+            with open(fin, "wb") as f:
+                f.write(np.array([M, N, 1], dtype="<i4").tobytes())
+                for arr in (an, bn, cn, am, bm, cm, y):
+                    f.write(np.asarray(arr, dtype="<f8").tobytes())
+            subprocess.run([str(exe), "blktri", str(fin), str(fout)], check=True, env=env1())
+            raw = fout.read_bytes()
+        ierr = int(np.frombuffer(raw, dtype="<i4", count=1)[0])
+        sol = np.frombuffer(raw, dtype="<f8", offset=4).reshape(N, M)
+        out[f"case{i}_grid"] = np.array([g.jmax, g.kmax, g.lmax, g.delr, g.delz, m], dtype=np.float64)
+        out[f"case{i}_y"] = y
+        out[f"case{i}_x"] = sol
+        out[f"case{i}_ierr"] = np.array(ierr)
+    np.savez_compressed(GOLDEN / "blktri_modes.npz", **out)
+
+
+# ---------------------------------------------------------------------------
+# Whole SETBDY + BDYGEN + POT3 solves
+
+SOLVE_CASES = [
+    # name, grid, density, (isym, maxtrm, npoint, redge), keep full coef
+    ("g16_small", fields.unit_grid(16, 8, 16), "small", (2, 10, 8, 0.0), True),
+    ("g16_m4_np5", fields.unit_grid(16, 8, 16), "small", (2, 4, 5, 0.0), True),
+    ("g16_isymneg_redge", fields.unit_grid(16, 8, 16), "small", (-2, 8, 8, 0.0), True),
+    ("g24x16x32_spiral", fields.unit_grid(24, 16, 32), "spiral", (2, 10, 8, 0.0), True),
+    ("g32_redge", fields.unit_grid(32, 16, 32), "small", (2, 10, 8, 0.9), True),
+    ("g32x16x64_aniso", fields.Grid(32, 16, 64, 1.0 / 32, 0.75 / 16), "spiral", (2, 6, 9, 0.0), True),
+    ("c1_64x32x64", fields.unit_grid(64, 32, 64), "small", (2, 10, 8, 0.0), False),
+]
+
+
+def density(g, kind):
+    if kind == "small":
+        return fields.density_small_torus(g, seed=1)
+    if kind == "spiral":
+        return fields.density_spiral_torus(g, seed=2)
+    raise ValueError(kind)
+
+
+def make_solves(only=None):
+    build(sorted({c[1].name for c in SOLVE_CASES}))
+    meta = {}
+    for name, g, kind, (isym, maxtrm, npoint, redge), fullcoef in SOLVE_CASES:
+        if only and name not in only:
+            continue
+        rho = density(g, kind)
+        res = run_solve(g, rho, isym, maxtrm, npoint, redge)
+        save = dict(
+            grid=np.array([g.jmax, g.kmax, g.lmax], dtype=np.int64),
+            spacing=np.array([g.delr, g.delz]),
+            params=np.array([isym, maxtrm, npoint], dtype=np.int64),
+            redge=np.array(redge),
+            phi=res["phi"],
+            phi_b_top=res["phi_b"][1:g.jmax + 1, g.kmax, :],      # K=KMAX+1, J=2..JMAX+1
+            phi_b_out=res["phi_b"][g.jmax, 1:g.kmax, :],          # J=JMAX+1, K=2..KMAX
+            phi_b_ghost=np.stack([res["phi_b"][g.jmax, 0, :], res["phi_b"][0, g.kmax, :]]),
+            tmass_b=np.array(res["tmass_b"]),
+            tmass=np.array(res["tmass"]),
+            phichk=np.array(res["phichk"]),
+            klocat=np.array(res["klocat"]),
+            coef_k2=res["coef"][:, 1, :, :],
+        )
+        if fullcoef:
+            save["coef"] = res["coef"]
+            save["rho"] = rho
+        meta[name] = dict(density=kind, rho_sha256=rho_digest(rho), isym=isym, maxtrm=maxtrm,
+                          npoint=npoint, redge=redge, grid=g.name, delr=g.delr, delz=g.delz)
+        np.savez_compressed(GOLDEN / f"solve_{name}.npz", **save)
+    mpath = GOLDEN / "solve_meta.json"
+    old = json.loads(mpath.read_text()) if mpath.exists() else {}
+    old.update(meta)
+    old["_generator"] = dict(
+        flang=subprocess.run(["flang", "--version"], capture_output=True, text=True).stdout.splitlines()[0],
+        flags="-O3 -fopenmp -cpp -mcmodel=medium -fdefault-real-8 -fdefault-double-8",
+        threads=1, numpy=np.__version__)
+    mpath.write_text(json.dumps(old, indent=1, sort_keys=True))
+
+
+def make_large(grid=(256, 128, 256), nrep=0):
+    """Slices of one large solve: planes K=2 and L=1, boundary rows, norms."""
+    g = fields.unit_grid(*grid)
+    build([g.name])
+    rho = fields.density_spiral_torus(g, seed=2)
+    res = run_solve(g, rho, 2, 10, 8, 0.0, nwarm=1 if nrep else 0, nrep=nrep)
+    phi = res["phi"]
+    # the same solve by the oracle's 80-bit build: the exact solution of the
+    # discrete system to ~1e-15, against which FP64 solvers are measured
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    pyoracle.use_extended(True)
+    ext = pyoracle.solve(g, rho)["phi"]
+    pyoracle.use_extended(False)
+    save = dict(grid=np.array(grid, dtype=np.int64), spacing=np.array([g.delr, g.delz]),
+                phi_k2=phi[:, 1, :], phi_l1=phi[:, :, 0],
+                phi_absmax=np.array(np.abs(phi).max()), phi_sum=np.array(phi.sum()),
+                phi_b_top=res["phi_b"][1:g.jmax + 1, g.kmax, :],
+                phi_b_out=res["phi_b"][g.jmax, 1:g.kmax, :],
+                tmass=np.array(res["tmass"]), phichk=np.array(res["phichk"]),
+                coef_k2=res["coef"][:, 1, :8, :],
+                ext_phi_k2=ext[:, 1, :], ext_phi_l1=ext[:, :, 0],
+                ext_ref_maxdiff=np.array(np.abs(ext - phi).max() / np.abs(ext).max()))
+    name = f"large_{g.name}"
+    np.savez_compressed(GOLDEN / f"solve_{name}.npz", **save)
+    mpath = GOLDEN / "solve_meta.json"
+    old = json.loads(mpath.read_text()) if mpath.exists() else {}
+    old[name] = dict(density="spiral", rho_sha256=rho_digest(rho), isym=2, maxtrm=10, npoint=8,
+                     redge=0.0, grid=g.name, delr=g.delr, delz=g.delz)
+    if nrep:
+        s = np.sort(res["seconds"])
+        old[name]["ref_cpu_seconds"] = dict(threads=res["threads"], p50=float(np.median(s)),
+                                            p99=float(s[min(len(s) - 1, int(0.99 * len(s)))]))
+    mpath.write_text(json.dumps(old, indent=1, sort_keys=True))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", default=None, help="fft | blktri | solve | large | <case name>")
+    args = ap.parse_args()
+    if not (pathlib.Path("/root/reference") / "pot").is_dir():
+        sys.exit("the reference tree is required to write golden fixtures")
+    if args.only in (None, "fft"):
+        make_fft()
+    if args.only in (None, "blktri"):
+        make_blktri()
+    if args.only in (None, "solve"):
+        make_solves()
+    elif args.only not in ("fft", "blktri", "large"):
+        make_solves(only={args.only})
+    if args.only in (None, "large"):
+        make_large((128, 64, 128))
+        make_large((256, 128, 256))
+
+
+if __name__ == "__main__":
+    main()
