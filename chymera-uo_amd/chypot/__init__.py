@@ -1,0 +1,237 @@
+"""Python host mirror of the MI355X potential solve (ctypes over libchypot.so).
+
+Mirrors the reference's operator interface for the self-gravity solve --
+``SETBDY(NCALL,ISYM)`` (pot/boundary.f:2), ``BDYGEN(MAXTRM,ISYM,REDGE)``
+(pot/boundary.f:88) and ``POT3(NPOINT,IPRINT)`` (pot/pot3.f:15) -- with explicit
+arrays in place of the COMMON blocks.  Arrays keep the reference's Fortran
+layout ``PHI(JMAX+2,KMAX+2,LMAX)``: a torch tensor of shape ``(LMAX, KMAX+2,
+JMAX+2)`` (C order) or a numpy array of shape ``(JMAX+2, KMAX+2, LMAX)`` with
+``order='F'`` has exactly that byte layout.
+
+Errors: every call raises :class:`ChyError` (carrying the library's code) where
+the reference would STOP or silently continue; nothing falls back to the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+from . import fields  # noqa: F401  (re-exported helper module)
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE.parent / "_lib" / "libchypot.so"
+
+CHY_OK, CHY_EINVAL, CHY_EUNSUPPORTED, CHY_ENOMEM, CHY_EDEVICE, CHY_ESTATE, CHY_ENUMERIC = range(7)
+COEF_NONE, COEF_K2, COEF_FULL = 0, 1, 2
+STAGES = ("bdygen", "fft_fwd", "zfwd", "rsolve", "zinv", "fft_inv", "ghost")
+
+# every entry point declared in include/chypot.h
+EXPORTS = (
+    "chy_ctx_create", "chy_ctx_destroy", "chy_ctx_set_stream", "chy_ctx_sync",
+    "chy_setbdy", "chy_grid_destroy", "chy_grid_device_bytes",
+    "chy_bdygen", "chy_pot3", "chy_solve", "chy_bdygen_host", "chy_pot3_host",
+    "chy_ctx_profile", "chy_ctx_stage_times",
+    "chy_strerror", "chy_last_device_error",
+)
+
+
+class ChyError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        msg = lib().chy_strerror(code).decode()
+        if code == CHY_EDEVICE:
+            msg += f" ({lib().chy_last_device_error().decode()})"
+        super().__init__(f"{where}: {msg} [code {code}]")
+        self.code = code
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("jmax", C.c_int), ("kmax", C.c_int), ("lmax", C.c_int),
+                ("r", C.POINTER(C.c_double)), ("z", C.POINTER(C.c_double)),
+                ("rhf", C.POINTER(C.c_double)), ("zhf", C.POINTER(C.c_double)),
+                ("dtheta", C.c_double), ("pi", C.c_double), ("grav", C.c_double)]
+
+
+class Diag(C.Structure):
+    _fields_ = [("tmass", C.c_double), ("phichk", C.c_double), ("klocat", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libchypot.so (raises if it was not built: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"libchypot.so not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        vp, dp = C.c_void_p, C.POINTER(C.c_double)
+        L.chy_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.chy_ctx_destroy.argtypes = [vp]
+        L.chy_ctx_set_stream.argtypes = [vp, vp]
+        L.chy_ctx_sync.argtypes = [vp]
+        L.chy_setbdy.argtypes = [vp, C.POINTER(GridDesc), C.c_int, C.POINTER(vp)]
+        L.chy_grid_destroy.argtypes = [vp]
+        L.chy_grid_device_bytes.argtypes = [vp]
+        L.chy_grid_device_bytes.restype = C.c_size_t
+        L.chy_bdygen.argtypes = [vp, C.c_int, C.c_int, C.c_double, vp, vp, C.POINTER(Diag)]
+        L.chy_pot3.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.POINTER(Diag)]
+        L.chy_solve.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp, C.c_int,
+                                C.POINTER(Diag)]
+        L.chy_bdygen_host.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, dp, C.POINTER(Diag)]
+        L.chy_pot3_host.argtypes = [vp, C.c_int, dp, dp, dp, C.c_int, C.POINTER(Diag)]
+        L.chy_ctx_profile.argtypes = [vp, C.c_int]
+        L.chy_ctx_stage_times.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int]
+        L.chy_strerror.argtypes = [C.c_int]
+        L.chy_strerror.restype = C.c_char_p
+        L.chy_last_device_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, where: str):
+    if rc != CHY_OK:
+        raise ChyError(rc, where)
+
+
+def _dptr(t):
+    """Device pointer of a torch CUDA tensor (float64, contiguous) or None."""
+    if t is None:
+        return None
+    if not t.is_cuda or t.dtype.__repr__() != "torch.float64" or not t.is_contiguous():
+        raise ValueError("expected a contiguous float64 CUDA tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One HIP device and the stream all work is issued on."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().chy_ctx_create(device, C.byref(h)), "chy_ctx_create")
+        self.handle = h
+        self.device = device
+
+    def set_stream(self, stream=None):
+        """Issue on a torch.cuda.Stream (or its raw handle); None = own stream."""
+        raw = getattr(stream, "cuda_stream", stream)
+        _check(lib().chy_ctx_set_stream(self.handle, C.c_void_p(raw) if raw else None),
+               "chy_ctx_set_stream")
+
+    def sync(self):
+        _check(lib().chy_ctx_sync(self.handle), "chy_ctx_sync")
+
+    def profile(self, enable: bool = True):
+        """Bracket every solver stage with HIP events (resets the totals)."""
+        _check(lib().chy_ctx_profile(self.handle, int(enable)), "chy_ctx_profile")
+
+    def stage_times(self):
+        """{stage: (total_ms, launches)} since profiling was enabled (waits)."""
+        ms = (C.c_double * len(STAGES))()
+        cnt = (C.c_int * len(STAGES))()
+        _check(lib().chy_ctx_stage_times(self.handle, ms, cnt, len(STAGES)), "chy_ctx_stage_times")
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(STAGES)}
+
+    def close(self):
+        if self.handle:
+            lib().chy_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PreparedGrid:
+    """SETBDY: a grid with its device-resident tables and workspace."""
+
+    def __init__(self, ctx: Context, grid, isym: int = 2, pi: float = fields.PI, grav: float = 1.0,
+                 r=None, z=None, rhf=None, zhf=None, dtheta=None):
+        self.ctx = ctx
+        self.grid = grid
+        self._arrs = [np.ascontiguousarray(a if a is not None else d(), dtype=np.float64)
+                      for a, d in ((r, grid.r), (z, grid.z), (rhf, grid.rhf), (zhf, grid.zhf))]
+        dp = C.POINTER(C.c_double)
+        desc = GridDesc(grid.jmax, grid.kmax, grid.lmax,
+                        *[a.ctypes.data_as(dp) for a in self._arrs],
+                        grid.dtheta if dtheta is None else dtheta, pi, grav)
+        h = C.c_void_p()
+        _check(lib().chy_setbdy(ctx.handle, C.byref(desc), isym, C.byref(h)), "chy_setbdy")
+        self.handle = h
+        self.isym = isym
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().chy_grid_device_bytes(self.handle))
+
+    # -- device-resident forms (torch CUDA float64 tensors) --
+    def bdygen(self, rho, phi, maxtrm=10, isym=None, redge=0.0, diag=False):
+        d = Diag() if diag else None
+        _check(lib().chy_bdygen(self.handle, maxtrm, self.isym if isym is None else isym, redge,
+                                _dptr(rho), _dptr(phi), C.byref(d) if d else None), "chy_bdygen")
+        return d
+
+    def pot3(self, rho, phi, npoint=8, coef=None, coef_mode=COEF_NONE, diag=None):
+        d = diag if isinstance(diag, Diag) else (Diag() if diag else None)
+        _check(lib().chy_pot3(self.handle, npoint, _dptr(rho), _dptr(phi), _dptr(coef), coef_mode,
+                              C.byref(d) if d else None), "chy_pot3")
+        return d
+
+    def solve(self, rho, phi, maxtrm=10, isym=None, redge=0.0, npoint=8, coef=None,
+              coef_mode=COEF_NONE, diag=None):
+        d = diag if isinstance(diag, Diag) else (Diag() if diag else None)
+        _check(lib().chy_solve(self.handle, maxtrm, self.isym if isym is None else isym, redge, npoint,
+                               _dptr(rho), _dptr(phi), _dptr(coef), coef_mode,
+                               C.byref(d) if d else None), "chy_solve")
+        return d
+
+    # -- host forms (numpy Fortran-ordered float64 arrays, updated in place) --
+    def bdygen_host(self, rho, phi, maxtrm=10, isym=None, redge=0.0):
+        _host_ok(rho, phi)
+        d = Diag()
+        dp = C.POINTER(C.c_double)
+        _check(lib().chy_bdygen_host(self.handle, maxtrm, self.isym if isym is None else isym, redge,
+                                     rho.ctypes.data_as(dp), phi.ctypes.data_as(dp), C.byref(d)),
+               "chy_bdygen_host")
+        return d
+
+    def pot3_host(self, rho, phi, npoint=8, coef=None, coef_mode=COEF_NONE, klocat=0):
+        _host_ok(rho, phi)
+        d = Diag(0.0, 0.0, klocat)
+        dp = C.POINTER(C.c_double)
+        cp = coef.ctypes.data_as(dp) if coef is not None else None
+        _check(lib().chy_pot3_host(self.handle, npoint, rho.ctypes.data_as(dp), phi.ctypes.data_as(dp),
+                                   cp, coef_mode, C.byref(d)), "chy_pot3_host")
+        return d
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().chy_grid_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _host_ok(*arrs):
+    for a in arrs:
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.f_contiguous):
+            raise ValueError("host arrays must be Fortran-ordered float64 numpy arrays")
+
+
+def to_device(a, device="cuda"):
+    """Fortran-ordered (J+2,K+2,L) numpy array -> torch CUDA tensor (L,K+2,J+2)."""
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to(device)
+
+
+def to_host(t):
+    """torch tensor (L,K+2,J+2) -> Fortran-ordered numpy array (J+2,K+2,L)."""
+    return np.asfortranarray(t.detach().cpu().numpy().T)

@@ -1,0 +1,381 @@
+// C-ABI of libchypot (include/chypot.h): context, SETBDY/BDYGEN/POT3 entry
+// points and their host-pointer forms.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "chypot_impl.h"
+
+namespace chy {
+
+static thread_local std::string g_last_error;
+
+void set_device_error(hipError_t e, const char *where)
+{
+    g_last_error = std::string(where) + ": " + hipGetErrorString(e);
+}
+
+static int dev_rc(int hip_rc, const char *where)
+{
+    if (hip_rc == (int)hipSuccess) return CHY_OK;
+    set_device_error((hipError_t)hip_rc, where);
+    return CHY_EDEVICE;
+}
+
+static void free_grid_memory(chy_grid *g)
+{
+    for (void *p : g->allocations) hipFree(p);
+    g->allocations.clear();
+    if (g->d.h_rho) hipFree(g->d.h_rho);
+    if (g->d.h_phi) hipFree(g->d.h_phi);
+    if (g->d.h_coef) hipFree(g->d.h_coef);
+    g->d.h_rho = g->d.h_phi = g->d.h_coef = nullptr;
+}
+
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev)
+    {
+        hipGetDevice(&prev);
+        if (prev != dev) hipSetDevice(dev);
+    }
+    ~DeviceScope()
+    {
+        int cur = -1;
+        hipGetDevice(&cur);
+        if (prev >= 0 && cur != prev) hipSetDevice(prev);
+    }
+};
+
+static size_t field_elems(const chy_grid *g) { return (size_t)(g->J + 2) * (g->K + 2) * g->L; }
+static size_t coef_elems(const chy_grid *g) { return (size_t)(g->J + 2) * (g->K + 2) * (g->L / 2) * 2; }
+
+static int check_pot3_args(const chy_grid *g, int &npoint, int coef_mode, const void *coef)
+{
+    if (npoint < 1) return CHY_EINVAL;
+    if (npoint > 10) npoint = 10;  // pot3.f:328-333
+    const int num = (npoint - npoint % 2) / 2;
+    // ZAXPHI reads PHI(J,...) for J up to NUM+1 (NUM+2 for odd NPOINT)
+    if (num + 1 + (npoint % 2) > g->J + 1) return CHY_EINVAL;
+    if (coef_mode < CHY_COEF_NONE || coef_mode > CHY_COEF_FULL) return CHY_EINVAL;
+    if (coef_mode != CHY_COEF_NONE && !coef) return CHY_EINVAL;
+    return CHY_OK;
+}
+
+// ---- stage profiler ----
+void Profiler::begin(int stage, hipStream_t s)
+{
+    if (!on) return;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, s);
+    open_stage = stage;
+    open_ev = e;
+}
+
+void Profiler::end(hipStream_t s)
+{
+    if (!on || open_stage < 0) return;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, s);
+    pending.push_back(Rec{open_stage, open_ev, e});
+    open_stage = -1;
+    open_ev = nullptr;
+}
+
+int Profiler::collect()
+{
+    int rc = CHY_OK;
+    for (Rec &r : pending) {
+        float t = 0.0f;
+        hipError_t e = hipEventSynchronize(r.b);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, r.a, r.b);
+        if (e == hipSuccess) {
+            ms[r.stage] += t;
+            count[r.stage] += 1;
+        } else {
+            rc = dev_rc((int)e, "stage timing");
+        }
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    pending.clear();
+    return rc;
+}
+
+void Profiler::reset()
+{
+    collect();
+    for (int i = 0; i < CHY_NSTAGES; ++i) { ms[i] = 0.0; count[i] = 0; }
+}
+
+static int run_bdygen(chy_grid *g, int maxtrm, int isym, double redge, const double *rho, double *phi)
+{
+    if (maxtrm < 0 || maxtrm > kMaxTrm || std::abs(isym) != 2 || !(redge >= 0.0)) return CHY_EINVAL;
+    Profiler &p = g->ctx->prof;
+    p.begin(CHY_STAGE_BDYGEN, g->ctx->stream);
+    int rc = dev_rc(launch_bdygen(g, maxtrm, isym, redge, rho, phi, g->ctx->stream), "bdygen launch");
+    p.end(g->ctx->stream);
+    return rc;
+}
+
+static int run_pot3(chy_grid *g, int npoint, const double *rho, double *phi, double *coef, int coef_mode,
+                    int klocat_in)
+{
+    hipStream_t s = g->ctx->stream;
+    Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
+    if (p) p->begin(CHY_STAGE_FFT_FWD, s);
+    int rc = dev_rc(launch_fft_forward(g, rho, phi, coef, coef_mode, s), "forward FFT launch");
+    if (p) p->end(s);
+    if (rc) return rc;
+    rc = dev_rc(launch_mode_solve(g, s, p), "mode solve launch");
+    if (rc) return rc;
+    if (p) p->begin(CHY_STAGE_FFT_INV, s);
+    rc = dev_rc(launch_fft_inverse(g, phi, s), "inverse FFT launch");
+    if (p) p->end(s);
+    if (rc) return rc;
+    if (p) p->begin(CHY_STAGE_GHOST, s);
+    rc = dev_rc(launch_ghosts_zaxphi(g, npoint, klocat_in, phi, s), "ghost/ZAXPHI launch");
+    if (p) p->end(s);
+    return rc;
+}
+
+static int read_diag(chy_grid *g, chy_diag *diag, bool tmass, bool zax)
+{
+    double h[3];
+    hipError_t e = hipMemcpyAsync(h, g->d.dscal, sizeof(h), hipMemcpyDeviceToHost, g->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+    if (e != hipSuccess) {
+        set_device_error(e, "diagnostics readback");
+        return CHY_EDEVICE;
+    }
+    if (tmass) diag->tmass = h[0];
+    if (zax) {
+        diag->phichk = h[1];
+        diag->klocat = (int)h[2];
+    }
+    return CHY_OK;
+}
+
+static int ensure_staging(chy_grid *g, bool coef)
+{
+    const size_t fb = sizeof(double) * field_elems(g);
+    if (!g->d.h_rho && hipMalloc(&g->d.h_rho, fb) != hipSuccess) return CHY_ENOMEM;
+    if (!g->d.h_phi && hipMalloc(&g->d.h_phi, fb) != hipSuccess) return CHY_ENOMEM;
+    if (coef && !g->d.h_coef && hipMalloc(&g->d.h_coef, sizeof(double) * coef_elems(g)) != hipSuccess)
+        return CHY_ENOMEM;
+    return CHY_OK;
+}
+
+}  // namespace chy
+
+using namespace chy;
+
+extern "C" {
+
+const char *chy_strerror(int code)
+{
+    switch (code) {
+    case CHY_OK: return "success";
+    case CHY_EINVAL: return "invalid argument or grid description";
+    case CHY_EUNSUPPORTED: return "unsupported grid (KMAX must be 2^k >= 4, LMAX a power of two <= 1024, JMAX <= 1024, |ISYM| = 2)";
+    case CHY_ENOMEM: return "out of memory";
+    case CHY_EDEVICE: return "HIP runtime error";
+    case CHY_ESTATE: return "call out of order";
+    case CHY_ENUMERIC: return "z operator not symmetrisable or eigen-solver did not converge";
+    default: return "unknown error";
+    }
+}
+
+const char *chy_last_device_error(void) { return g_last_error.c_str(); }
+
+int chy_ctx_create(int device, chy_ctx **out)
+{
+    if (!out) return CHY_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return CHY_EINVAL;
+    chy_ctx *c = new (std::nothrow) chy_ctx;
+    if (!c) return CHY_ENOMEM;
+    c->device = device;
+    DeviceScope ds(device);
+    hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        set_device_error(e, "hipStreamCreate");
+        delete c;
+        return CHY_EDEVICE;
+    }
+    c->stream = c->own;
+    *out = c;
+    return CHY_OK;
+}
+
+int chy_ctx_destroy(chy_ctx *ctx)
+{
+    if (!ctx) return CHY_EINVAL;
+    DeviceScope ds(ctx->device);
+    ctx->prof.collect();
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+    return CHY_OK;
+}
+
+int chy_ctx_profile(chy_ctx *ctx, int enable)
+{
+    if (!ctx) return CHY_EINVAL;
+    DeviceScope ds(ctx->device);
+    ctx->prof.reset();
+    ctx->prof.on = enable != 0;
+    return CHY_OK;
+}
+
+int chy_ctx_stage_times(chy_ctx *ctx, double *ms, int *counts, int n)
+{
+    if (!ctx || n < 0 || (n > 0 && (!ms || !counts))) return CHY_EINVAL;
+    DeviceScope ds(ctx->device);
+    int rc = ctx->prof.collect();
+    for (int i = 0; i < n && i < CHY_NSTAGES; ++i) {
+        ms[i] = ctx->prof.ms[i];
+        counts[i] = ctx->prof.count[i];
+    }
+    return rc;
+}
+
+int chy_ctx_set_stream(chy_ctx *ctx, void *stream)
+{
+    if (!ctx) return CHY_EINVAL;
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own;
+    return CHY_OK;
+}
+
+int chy_ctx_sync(chy_ctx *ctx)
+{
+    if (!ctx) return CHY_EINVAL;
+    DeviceScope ds(ctx->device);
+    return dev_rc((int)hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int chy_setbdy(chy_ctx *ctx, const chy_grid_desc *desc, int isym, chy_grid **out)
+{
+    if (!ctx || !desc || !out) return CHY_EINVAL;
+    *out = nullptr;
+    chy_grid *g = new (std::nothrow) chy_grid;
+    if (!g) return CHY_ENOMEM;
+    g->ctx = ctx;
+    DeviceScope ds(ctx->device);
+    int rc = build_grid(g, desc, isym);
+    if (rc != CHY_OK) {
+        free_grid_memory(g);
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return CHY_OK;
+}
+
+int chy_grid_destroy(chy_grid *g)
+{
+    if (!g) return CHY_EINVAL;
+    DeviceScope ds(g->ctx->device);
+    hipStreamSynchronize(g->ctx->stream);
+    free_grid_memory(g);
+    delete g;
+    return CHY_OK;
+}
+
+size_t chy_grid_device_bytes(const chy_grid *g) { return g ? g->device_bytes : 0; }
+
+int chy_bdygen(chy_grid *g, int maxtrm, int isym, double redge, const double *rho_d, double *phi_d,
+               chy_diag *diag)
+{
+    if (!g) return CHY_ESTATE;  // BDYGEN without SETBDY (boundary.f:311)
+    if (!rho_d || !phi_d) return CHY_EINVAL;
+    DeviceScope ds(g->ctx->device);
+    int rc = run_bdygen(g, maxtrm, isym, redge, rho_d, phi_d);
+    if (rc == CHY_OK && diag) rc = read_diag(g, diag, true, false);
+    return rc;
+}
+
+int chy_pot3(chy_grid *g, int npoint, const double *rho_d, double *phi_d, double *coef_d, int coef_mode,
+             chy_diag *diag)
+{
+    if (!g) return CHY_ESTATE;
+    if (!rho_d || !phi_d) return CHY_EINVAL;
+    int rc = check_pot3_args(g, npoint, coef_mode, coef_d);
+    if (rc) return rc;
+    DeviceScope ds(g->ctx->device);
+    rc = run_pot3(g, npoint, rho_d, phi_d, coef_d, coef_mode, diag ? diag->klocat : 0);
+    if (rc == CHY_OK && diag) rc = read_diag(g, diag, false, true);
+    return rc;
+}
+
+int chy_solve(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho_d,
+              double *phi_d, double *coef_d, int coef_mode, chy_diag *diag)
+{
+    if (!g) return CHY_ESTATE;
+    if (!rho_d || !phi_d) return CHY_EINVAL;
+    int rc = check_pot3_args(g, npoint, coef_mode, coef_d);
+    if (rc) return rc;
+    DeviceScope ds(g->ctx->device);
+    rc = run_bdygen(g, maxtrm, isym, redge, rho_d, phi_d);
+    if (rc) return rc;
+    rc = run_pot3(g, npoint, rho_d, phi_d, coef_d, coef_mode, diag ? diag->klocat : 0);
+    if (rc == CHY_OK && diag) rc = read_diag(g, diag, true, true);
+    return rc;
+}
+
+int chy_bdygen_host(chy_grid *g, int maxtrm, int isym, double redge, const double *rho_h, double *phi_h,
+                    chy_diag *diag)
+{
+    if (!g) return CHY_ESTATE;
+    if (!rho_h || !phi_h) return CHY_EINVAL;
+    DeviceScope ds(g->ctx->device);
+    int rc = ensure_staging(g, false);
+    if (rc) return rc;
+    hipStream_t s = g->ctx->stream;
+    const size_t fb = sizeof(double) * field_elems(g);
+    hipError_t e = hipMemcpyAsync(g->d.h_rho, rho_h, fb, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->d.h_phi, phi_h, fb, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return dev_rc((int)e, "host to device copy");
+    rc = run_bdygen(g, maxtrm, isym, redge, g->d.h_rho, g->d.h_phi);
+    if (rc) return rc;
+    e = hipMemcpyAsync(phi_h, g->d.h_phi, fb, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return dev_rc((int)e, "device to host copy");
+    return diag ? read_diag(g, diag, true, false) : CHY_OK;
+}
+
+int chy_pot3_host(chy_grid *g, int npoint, const double *rho_h, double *phi_h, double *coef_h, int coef_mode,
+                  chy_diag *diag)
+{
+    if (!g) return CHY_ESTATE;
+    if (!rho_h || !phi_h) return CHY_EINVAL;
+    int rc = check_pot3_args(g, npoint, coef_mode, coef_h);
+    if (rc) return rc;
+    DeviceScope ds(g->ctx->device);
+    rc = ensure_staging(g, coef_mode != CHY_COEF_NONE);
+    if (rc) return rc;
+    hipStream_t s = g->ctx->stream;
+    const size_t fb = sizeof(double) * field_elems(g);
+    const size_t cb = sizeof(double) * coef_elems(g);
+    hipError_t e = hipMemcpyAsync(g->d.h_rho, rho_h, fb, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->d.h_phi, phi_h, fb, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && coef_mode != CHY_COEF_NONE)
+        e = hipMemcpyAsync(g->d.h_coef, coef_h, cb, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return dev_rc((int)e, "host to device copy");
+    rc = run_pot3(g, npoint, g->d.h_rho, g->d.h_phi, g->d.h_coef, coef_mode, diag ? diag->klocat : 0);
+    if (rc) return rc;
+    e = hipMemcpyAsync(phi_h, g->d.h_phi, fb, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && coef_mode != CHY_COEF_NONE)
+        e = hipMemcpyAsync(coef_h, g->d.h_coef, cb, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return dev_rc((int)e, "device to host copy");
+    return diag ? read_diag(g, diag, false, true) : CHY_OK;
+}
+
+}  // extern "C"

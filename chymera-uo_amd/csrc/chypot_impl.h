@@ -1,0 +1,134 @@
+// Internal declarations of libchypot: per-grid state, device tables and the
+// kernel launchers.  Not part of the C-ABI (include/chypot.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "chypot.h"
+
+namespace chy {
+
+// Largest multipole order BDYGEN supports (MAXTRM <= 10, pot/boundary.f:94-96)
+// and the number of (l,m) terms with l+m even up to it (the ISYM=2 set,
+// pot/boundary.f:345-354,412-500).
+constexpr int kMaxTrm = 10;
+constexpr int kMaxTerms = 36;
+// Ring moments per ring: SUMASS, CM(1..10), SM(1..10) (pot/boundary.f:526-538).
+constexpr int kMoments = 21;
+// Rings per boundary chunk (the unit of the deterministic radial reduction).
+constexpr int kChunkRings = 64;
+
+struct Term {
+    int l, m;
+    double dlm;       // boundary factor DLM (pot/boundary.f:694-772), REAL*8 TW constants
+    int np;           // number of polynomial coefficients
+    double poly[6];   // BLM(c,s) = s^m * sum_i poly[i] c^(2i) (pot/boundary.f:204-253)
+};
+const Term *terms_table(int *count);  // the kMaxTerms terms, l-major, m ascending
+
+// Device-side grid tables and workspace.  All pointers are device memory.
+struct DeviceGrid {
+    // FFT
+    double2 *twL = nullptr;      // (cos, sin)(2 pi e / L), e = 0..L-1
+    double2 *tw64 = nullptr;     // (cos, sin)(2 pi e / 64), e = 0..63 (in-register DFTs)
+    // z eigenbasis (BLKTRI's m-independent z operator, diagonalised)
+    double *wf = nullptr;        // [NZ][PP]   Y -> eigen coordinates:  Yh[p] = sum_k wf[k][p] Y[k]
+    double *wi = nullptr;        // [NZ][PP]   eigen coordinates -> X:  X[k] = sum_p wi[k][p] Z[p]
+    double *lam = nullptr;       // [PP]       eigenvalues (padded entries 1)
+    // r operator (pot3.f:172-194,226-230): AM(1)=0, CM(M)=0, BM per azimuthal order m
+    double *am = nullptr;        // [M]
+    double *cm = nullptr;        // [M]
+    double *bm = nullptr;        // [(N+1)][M]
+    // ZAXPHI abscissae (pot3.f:342-348)
+    double *zx = nullptr;        // [11]
+    // BDYGEN
+    int *ring_of_pos = nullptr;  // [nring]   sorted position -> ring index (jj-2)+(J-1)(kk-2)
+    double *tin = nullptr;       // [nring][nt]  BLM(theta) r^l at sorted position
+    double *tout = nullptr;      // [nring][nt]  BLM(theta) r^-(l+1)  (= tin * r^-(2l+1))
+    double *rsph = nullptr;      // [nring]   spherical radius at sorted position
+    double *dv = nullptr;        // [nring]   DV by ring index (boundary.f:516)
+    int *chunk_start = nullptr;  // [nchunk+1] chunk boundaries in sorted positions
+    int *jk_cut = nullptr;       // [jkmax]   chunks 0..cut-1 are inside boundary point jk
+    double *tb = nullptr;        // [jkmax][nt] DLM*BLM(theta_b) (boundary.f:670-772)
+    double *rb = nullptr;        // [jkmax]
+    int *jpos = nullptr, *kpos = nullptr;
+    double *cosm = nullptr;      // [L][10] cos(m psi_L), psi_L=(L-1/2) DTHETA (boundary.f:31-37)
+    double *sinm = nullptr;      // [L][10]
+    // workspace
+    double *spec = nullptr;      // [L][KMAX][JMAX]  azimuthal spectrum, lanes J=2..JMAX+1, K=2..KMAX+1
+    double *yh = nullptr;        // [L][PP][JP]      z-eigen coordinates of the mode RHS / solution
+    double *mom = nullptr;       // [nring][kMoments]
+    double *chunk = nullptr;     // [nchunk][2][2*kMaxTerms]  inside / outside chunk sums
+    double *massio = nullptr;    // [nchunk+1][2][2*kMaxTerms] exclusive prefix (in) / suffix (out)
+    double *dscal = nullptr;     // [4 + KMAX+2]: tmass, phichk, klocat, _, phac[K]
+    // staging for the host-pointer entry points (allocated on first use)
+    double *h_rho = nullptr, *h_phi = nullptr, *h_coef = nullptr;
+};
+
+}  // namespace chy
+
+namespace chy {
+// HIP-event brackets around solver stages (enabled by chy_ctx_profile).
+struct Profiler {
+    bool on = false;
+    struct Rec { int stage; hipEvent_t a, b; };
+    std::vector<Rec> pending;
+    double ms[CHY_NSTAGES] = {};
+    int count[CHY_NSTAGES] = {};
+    int open_stage = -1;
+    hipEvent_t open_ev = nullptr;
+    void begin(int stage, hipStream_t s);
+    void end(hipStream_t s);
+    int collect();  // wait for pending records and accumulate
+    void reset();
+};
+}  // namespace chy
+
+struct chy_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    chy::Profiler prof;
+};
+
+struct chy_grid {
+    chy_ctx *ctx = nullptr;
+    int J = 0, K = 0, L = 0;     // JMAX, KMAX, LMAX
+    int N = 0;                   // LMAX/2
+    int M = 0, NZ = 0;           // JMAX-1, KMAX-1: interior r and z unknowns of a mode
+    int PP = 0;                  // NZ padded to a multiple of 64
+    int JP = 0;                  // M padded to a multiple of 64
+    int isym = 2;
+    double dtheta = 0, pi = 0, grav = 0;
+    double cmmax = 0, cnmax = 0, ammin = 0, pig4 = 0;
+    int nring = 0, nchunk = 0, jkmax = 0;
+    std::vector<double> r, z, rhf, zhf;
+    size_t device_bytes = 0;
+    std::vector<void *> allocations;
+    chy::DeviceGrid d;
+};
+
+namespace chy {
+
+// host-side preparation (plan.cpp)
+int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym);
+int symmetric_tridiag_eigen(int n, std::vector<long double> &d, std::vector<long double> &e,
+                            std::vector<long double> &q);
+
+// kernel launchers (one .hip file each); all return hipError_t as int
+int launch_fft_forward(const chy_grid *g, const double *rho, const double *phi, double *coef,
+                       int coef_mode, hipStream_t s);
+int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s);
+int launch_mode_solve(const chy_grid *g, hipStream_t s, Profiler *prof);
+int launch_ghosts_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s);
+int launch_bdygen(const chy_grid *g, int maxtrm, int isym, double redge, const double *rho,
+                  double *phi, hipStream_t s);
+
+void set_device_error(hipError_t e, const char *where);
+
+}  // namespace chy

@@ -1,0 +1,165 @@
+// POT3 epilogue: ghost cells (pot/pot3.f:287-298) and ZAXPHI
+// (pot/pot3.f:311-399), after the inverse FFT has written J=2..JMAX+1,
+// K=2..KMAX+1.
+//   k_axis_zaxphi  one block per K=2..KMAX+1: the axis ghost
+//                  PHI(1,K,L)=PHI(2,K,L+-N) (the cell across the axis), the
+//                  NPOINT Aitken extrapolation of PHI to r=0 from
+//                  x=+-RHF(2..) written to PHI(JMAX+2,K,L) and
+//                  PHI(JMAX+2,K,L+N), and PHAC(K), the largest azimuthal
+//                  spread of that value (first index wins ties, as the
+//                  reference's sequential strict-greater scans do)
+//   k_midplane     PHI(J,1,L)=PHI(J,2,L) for J=1..JMAX+1 (the J=1 entry is
+//                  the axis ghost of K=2, taken at its source)
+//   k_phichk       PHICHK/KLOCAT: the PHAC of largest magnitude, first K wins
+#include "chypot_impl.h"
+
+namespace chy {
+namespace {
+
+struct GhostArgs {
+    double *phi;
+    const double *zx;  // RHF(1..11)
+    double *dscal;     // [0]=tmass [1]=phichk [2]=klocat [4+K]=PHAC(K)
+    int J, K, L, JS, KS, npoint;
+    int klocat_in;     // KLOCAT before the call (kept when every PHAC is 0)
+};
+
+// argmax of |v| with the smallest index winning ties, across one block
+__device__ void block_argmax_abs(double v, int idx, double *sv, int *si, double &out_v, int &out_i)
+{
+    const int tid = threadIdx.x;
+    sv[tid] = v;
+    si[tid] = idx;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            const double a = sv[tid], b = sv[tid + s];
+            const int ia = si[tid], ib = si[tid + s];
+            const bool takeb = (fabs(b) > fabs(a)) || (fabs(b) == fabs(a) && ib < ia);
+            if (takeb) { sv[tid] = b; si[tid] = ib; }
+        }
+        __syncthreads();
+    }
+    out_v = sv[0];
+    out_i = si[0];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_axis_zaxphi(GhostArgs a)
+{
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    const int K = blockIdx.x + 2;  // Fortran K
+    const int tid = threadIdx.x;
+    const int N = a.L / 2;
+    const size_t plane = (size_t)a.JS * a.KS;
+    const size_t row = (size_t)a.JS * (K - 1);
+    double *phi = a.phi;
+
+    // axis ghost J=1
+    for (int l = tid; l < a.L; l += blockDim.x) {
+        const int lp = l < N ? l + N : l - N;
+        phi[row + plane * l] = phi[row + 1 + plane * lp];
+    }
+
+    // ZAXPHI abscissae (pot3.f:336-348)
+    const int np = a.npoint;
+    const int ichk = np % 2, nn = np - ichk, num = nn / 2, ispecl = num + num + 1, jsp = num + 1;
+    double x[11];
+    for (int jj = 2; jj <= jsp; ++jj) {
+        x[num + (jj - 1)] = a.zx[jj - 1];
+        x[num - (jj - 2)] = -a.zx[jj - 1];
+    }
+    if (ichk == 1) x[ispecl] = a.zx[num + 1];
+
+    // extrapolated value PH(L) for L=1..N (pot3.f:354-379)
+    double best = 0.0;
+    int besti = 0x7fffffff;
+    for (int l = tid; l < N; l += blockDim.x) {
+        double p2[11];
+        for (int jj = 2; jj <= jsp; ++jj) {
+            p2[num + (jj - 1)] = phi[row + (jj - 1) + plane * l];
+            p2[num - (jj - 2)] = phi[row + (jj - 1) + plane * (l + N)];
+        }
+        if (ichk == 1) p2[ispecl] = phi[row + (num + 1) + plane * l];
+        for (int i = 1; i <= np - 1; ++i) {
+            const double p1 = p2[i], xinv = 1.0 / x[i], xr = 0.0 * xinv;
+            for (int jj = i + 1; jj <= np; ++jj) {
+                const double xratio = x[jj] * xinv, pb = p2[jj];
+                p2[jj] = (p1 * (xratio - xr) + pb * (xr - 1.0)) / (xratio - 1.0);
+            }
+        }
+        const double ph = p2[np];
+        phi[row + (a.J + 1) + plane * l] = ph;
+        phi[row + (a.J + 1) + plane * (l + N)] = ph;
+        if (fabs(ph) > fabs(best) || (fabs(ph) == fabs(best) && l < besti)) { best = ph; besti = l; }
+    }
+    double phmax;
+    int imax;
+    block_argmax_abs(best, besti, sv, si, phmax, imax);
+    // largest |1 - PH/PHMAX| (pot3.f:381-386); PH re-read from what this block wrote
+    __threadfence_block();
+    double err = 0.0;
+    int erri = 0x7fffffff;
+    for (int l = tid; l < N; l += blockDim.x) {
+        const double ph = phi[row + (a.J + 1) + plane * l];
+        const double er = 1.0 - ph / phmax;
+        if (fabs(er) > fabs(err) || (fabs(er) == fabs(err) && fabs(er) > 0.0 && l < erri)) { err = er; erri = l; }
+    }
+    double e;
+    int ei;
+    block_argmax_abs(err, erri, sv, si, e, ei);
+    if (tid == 0) a.dscal[4 + K] = e;
+}
+
+__global__ __launch_bounds__(256) void k_midplane(GhostArgs a)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)(a.J + 1) * a.L;
+    if (idx >= total) return;
+    const int jf = (int)(idx % (a.J + 1)) + 1;  // Fortran J = 1..JMAX+1
+    const int l = (int)(idx / (a.J + 1));
+    const int N = a.L / 2;
+    const size_t plane = (size_t)a.JS * a.KS;
+    double v;
+    if (jf == 1) {
+        const int lp = l < N ? l + N : l - N;
+        v = a.phi[(size_t)1 + (size_t)a.JS + plane * lp];  // PHI(2,2,L+-N)
+    } else {
+        v = a.phi[(size_t)(jf - 1) + (size_t)a.JS + plane * l];
+    }
+    a.phi[(size_t)(jf - 1) + plane * l] = v;
+}
+
+__global__ void k_phichk(GhostArgs a)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double err = 0.0;
+    double kl = (double)a.klocat_in;
+    for (int K = 2; K <= a.K + 1; ++K) {
+        const double v = a.dscal[4 + K];
+        if (fabs(v) > fabs(err)) { err = v; kl = (double)K; }
+    }
+    a.dscal[1] = err;
+    a.dscal[2] = kl;
+}
+
+}  // namespace
+
+int launch_ghosts_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s)
+{
+    GhostArgs a{};
+    a.klocat_in = klocat_in;
+    a.phi = phi;
+    a.zx = g->d.zx;
+    a.dscal = g->d.dscal;
+    a.J = g->J; a.K = g->K; a.L = g->L; a.JS = g->J + 2; a.KS = g->K + 2;
+    a.npoint = npoint;
+    hipLaunchKernelGGL(k_axis_zaxphi, dim3(g->K), dim3(256), 0, s, a);
+    const long total = (long)(g->J + 1) * g->L;
+    hipLaunchKernelGGL(k_midplane, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_phichk, dim3(1), dim3(64), 0, s, a);
+    return (int)hipGetLastError();
+}
+
+}  // namespace chy

@@ -1,0 +1,164 @@
+/*
+ * chypot.h -- C-ABI of the MI355X self-gravity potential solve (libchypot.so).
+ *
+ * Drop-in boundary for CHYMERA's potential solve.  The reference exposes it as
+ * three Fortran subroutines that exchange data through COMMON blocks:
+ *
+ *   SUBROUTINE SETBDY(NCALL,ISYM)          pot/boundary.f:2
+ *   SUBROUTINE BDYGEN(MAXTRM,ISYM,REDGE)   pot/boundary.f:88
+ *   SUBROUTINE POT3(NPOINT,IPRINT)         pot/pot3.f:15   (calls ZAXPHI, pot/pot3.f:311)
+ *
+ * with PHI/RHO in /POIS/ (headers/globals.h:55-58), the grid in /GRID/
+ * (headers/globals.h:104-109), DTHETA/PI/GRAV in /blok6/ (headers/globals.h:8-10),
+ * COEF in /COEFS/ (pot/pot3.f:20) and TMASS/PHICHK/KLOCAT in /INSIDE/
+ * (pot/pot3.f:317).  Here the same operations take explicit pointers instead of
+ * COMMON storage; the Fortran shims in chymera-uo_amd/fortran/ map the COMMON
+ * blocks onto these calls.
+ *
+ * Array layout is the reference's Fortran layout, unchanged:
+ *   phi, rho : PHI(JMAX+2, KMAX+2, LMAX)      J fastest (element (J,K,L) at
+ *              (J-1) + (JMAX+2)*((K-1) + (KMAX+2)*(L-1)), 1-based J,K,L)
+ *   coef     : COEF(JMAX+2, KMAX+2, LMAX/2, 2)
+ *   r, rhf   : length JMAX+2;  z, zhf : length KMAX+2   (host memory)
+ *
+ * Every entry point returns 0 on success or one of the CHY_E* codes below;
+ * chy_strerror() gives the message.  The reference stops the program on the
+ * errors it detects (pot/boundary.f:311,901-907; pot/fft.f:625-627) and
+ * ignores BLKTRI's IERROR (pot/pot3.f:234-239); here every such condition is a
+ * returned error and nothing is computed.
+ *
+ * Pointers named *_d are device pointers (HIP device memory on the context's
+ * device); *_h are host pointers.  Calls on one context are issued on that
+ * context's HIP stream in order; the *_d entry points return without waiting
+ * for the GPU unless stated.
+ */
+#ifndef CHYPOT_H
+#define CHYPOT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CHY_OK = 0,
+    CHY_EINVAL = 1,       /* bad argument or grid description */
+    CHY_EUNSUPPORTED = 2, /* grid the solver cannot take: KMAX not 2^k (BLKTRI's own
+                             precondition, pot/blktri.f:29-31), LMAX not a power of
+                             two in [2,1024], |ISYM| != 2 */
+    CHY_ENOMEM = 3,       /* device or host allocation failed */
+    CHY_EDEVICE = 4,      /* HIP runtime error */
+    CHY_ESTATE = 5,       /* call out of order (e.g. BDYGEN before SETBDY,
+                             pot/boundary.f:311) */
+    CHY_ENUMERIC = 6      /* z operator not symmetrisable (AN(k)*CN(k-1) <= 0) or
+                             eigen-solver failed to converge */
+};
+
+/* Grid as the reference holds it in /GRID/ and /blok6/. */
+typedef struct {
+    int jmax, kmax, lmax;      /* JMAX, KMAX, LMAX (headers/hydroparam.h:16-20) */
+    const double *r, *z;       /* R(JMAX+2), Z(KMAX+2)   cell faces   (host) */
+    const double *rhf, *zhf;   /* RHF(JMAX+2), ZHF(KMAX+2) cell centres (host) */
+    double dtheta, pi, grav;   /* DTHETA, PI, GRAV */
+} chy_grid_desc;
+
+/* /INSIDE/ diagnostics written by BDYGEN (TMASS) and ZAXPHI (PHICHK, KLOCAT). */
+typedef struct {
+    double tmass;   /* total mass of the rings BDYGEN integrated (boundary.f:554) */
+    double phichk;  /* largest azimuthal spread of the z-axis potential (pot3.f:389-399) */
+    int klocat;     /* K where it occurs; unchanged when every spread is 0 */
+} chy_diag;
+
+/* What POT3's COEF output covers (pot3.f:105-132).  The hydro loop reads only
+ * the K=2 plane (radhydro/3dhyd-main.f:708,784), so the plane alone saves a
+ * full-field write per solve. */
+enum { CHY_COEF_NONE = 0, CHY_COEF_K2 = 1, CHY_COEF_FULL = 2 };
+
+typedef struct chy_ctx chy_ctx;
+typedef struct chy_grid chy_grid;
+
+/* ---- context (no reference counterpart: the reference runs in-process) ---- */
+
+/* Context on HIP device `device`; owns a stream unless one is set. */
+int chy_ctx_create(int device, chy_ctx **out);
+int chy_ctx_destroy(chy_ctx *ctx);
+/* Issue all later work of this context on `stream` (a hipStream_t; NULL = the
+ * context's own stream). */
+int chy_ctx_set_stream(chy_ctx *ctx, void *stream);
+/* Wait for all work issued on the context's stream. */
+int chy_ctx_sync(chy_ctx *ctx);
+
+/* ---- SETBDY (pot/boundary.f:2-82) ----
+ * Builds and uploads everything that depends only on the grid: the boundary
+ * point list and cos/sin tables of SETBDY, BLKTRI's m-independent z-operator
+ * data (here: its eigenbasis), the radial coefficients of pot3.f:158-194, FFT
+ * twiddles and the workspace.  ISYM as in SETBDY; only |ISYM|=2 (equatorial
+ * symmetry) is supported, the geometry POT3's midplane condition assumes. */
+int chy_setbdy(chy_ctx *ctx, const chy_grid_desc *grid, int isym, chy_grid **out);
+int chy_grid_destroy(chy_grid *g);
+/* Device bytes held by a prepared grid (tables + workspace). */
+size_t chy_grid_device_bytes(const chy_grid *g);
+
+/* ---- BDYGEN (pot/boundary.f:88-908) ----
+ * Multipole boundary potential (l <= maxtrm <= 10) from rho_d into the boundary
+ * cells of phi_d: K=KMAX+1 for J=2..JMAX+1, J=JMAX+1 for K=2..KMAX, and the
+ * ISYM=2 ghosts PHI(JMAX+1,1,L) and PHI(1,KMAX+1,L).  Nothing else in phi_d is
+ * written.  redge > 0 skips mass beyond spherical radius redge
+ * (boundary.f:379).  diag (host, may be NULL) receives tmass after the call
+ * has completed on the GPU (the call then waits). */
+int chy_bdygen(chy_grid *g, int maxtrm, int isym, double redge, const double *rho_d,
+               double *phi_d, chy_diag *diag);
+
+/* ---- POT3 + ZAXPHI (pot/pot3.f:15-423) ----
+ * Solves for phi_d given rho_d and the boundary values BDYGEN left in phi_d.
+ * Writes PHI on J=1..JMAX+2 x K=1..KMAX+1 exactly as the reference does (the
+ * K=KMAX+2 row and PHI(JMAX+2,1,L) are left as they were), and COEF per
+ * coef_mode into coef_d (same full COEF shape; may be NULL for CHY_COEF_NONE;
+ * amplitudes are left unchanged in lanes whose mean is exactly 0, as
+ * pot3.f:106 does).  npoint is ZAXPHI's NPOINT (1..10; larger is clamped to
+ * 10 as pot3.f:328-333 does).  diag (host, may be NULL) receives phichk and
+ * klocat; klocat is read from diag first so that it stays unchanged when every
+ * spread is 0 (pot3.f:391-398).  When diag is non-NULL the call waits. */
+int chy_pot3(chy_grid *g, int npoint, const double *rho_d, double *phi_d, double *coef_d,
+             int coef_mode, chy_diag *diag);
+
+/* BDYGEN followed by POT3: one potential solve, device-resident, one stream. */
+int chy_solve(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho_d,
+              double *phi_d, double *coef_d, int coef_mode, chy_diag *diag);
+
+/* Host-pointer forms used by the Fortran shims: rho_h/phi_h/coef_h are the
+ * COMMON arrays.  They copy to the grid's device buffers, run, copy back and
+ * wait.  phi_h is read (boundary values, untouched cells) and written. */
+int chy_bdygen_host(chy_grid *g, int maxtrm, int isym, double redge, const double *rho_h,
+                    double *phi_h, chy_diag *diag);
+int chy_pot3_host(chy_grid *g, int npoint, const double *rho_h, double *phi_h, double *coef_h,
+                  int coef_mode, chy_diag *diag);
+
+/* ---- stage timing (no reference counterpart; for benchmarks) ----
+ * While enabled, every stage of chy_bdygen/chy_pot3/chy_solve is bracketed by
+ * HIP events on the context's stream.  chy_ctx_stage_times() waits for the
+ * stream and returns, per stage, the summed milliseconds and the number of
+ * launches since profiling was (re)enabled. */
+enum {
+    CHY_STAGE_BDYGEN = 0, /* ring moments, radial split, boundary evaluation */
+    CHY_STAGE_FFT_FWD,    /* forward azimuthal RFFT (+ COEF) */
+    CHY_STAGE_ZFWD,       /* mode right-hand sides -> z eigen coordinates */
+    CHY_STAGE_RSOLVE,     /* r-tridiagonal solves */
+    CHY_STAGE_ZINV,       /* z eigen coordinates -> mode solutions */
+    CHY_STAGE_FFT_INV,    /* inverse azimuthal RFFT */
+    CHY_STAGE_GHOST,      /* ghosts, ZAXPHI, PHICHK */
+    CHY_NSTAGES
+};
+int chy_ctx_profile(chy_ctx *ctx, int enable);
+int chy_ctx_stage_times(chy_ctx *ctx, double *ms, int *counts, int n);
+
+const char *chy_strerror(int code);
+/* Last HIP error text seen by the library (for CHY_EDEVICE). */
+const char *chy_last_device_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHYPOT_H */

@@ -1,0 +1,138 @@
+"""GPU parity of the potential solve against the reference's golden vectors.
+
+Every test calls the HIP library through its C-ABI (chypot ctypes bindings)
+and compares with fixtures written by the compiled reference
+(tests/golden/make_golden.py) or with the oracle under oracle/.
+"""
+import numpy as np
+import pytest
+
+import chypot
+from chypot import fields
+from helpers import GOLDEN, SMALL_CASES, load_case, phase_diff, rel_max
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return chypot.Context(0)
+
+
+def run_solve(ctx, g, rho, params, coef_mode=chypot.COEF_FULL, phi_init=0.0):
+    pg = chypot.PreparedGrid(ctx, g, isym=params["isym"])
+    rho_d = chypot.to_device(rho)
+    phi_d = torch.full_like(rho_d, phi_init)
+    coef = None
+    if coef_mode != chypot.COEF_NONE:
+        coef = torch.zeros((2, g.lmax // 2, g.kmax + 2, g.jmax + 2), dtype=torch.float64, device="cuda")
+    pg.bdygen(rho_d, phi_d, maxtrm=params["maxtrm"], isym=params["isym"], redge=params["redge"])
+    ctx.sync()
+    phib = chypot.to_host(phi_d)
+    d_b = pg.bdygen(rho_d, phi_d, maxtrm=params["maxtrm"], isym=params["isym"], redge=params["redge"],
+                    diag=True)
+    d = pg.pot3(rho_d, phi_d, npoint=params["npoint"], coef=coef, coef_mode=coef_mode, diag=True)
+    ctx.sync()
+    out = dict(phi=chypot.to_host(phi_d), phi_b=phib, tmass=d_b.tmass, phichk=d.phichk, klocat=d.klocat)
+    if coef is not None:
+        out["coef"] = np.asfortranarray(coef.cpu().numpy().transpose(3, 2, 1, 0))
+    pg.close()
+    return out
+
+
+@pytest.mark.parametrize("name", SMALL_CASES)
+def test_solve_matches_reference(ctx, name):
+    g, rho, params, f = load_case(name)
+    print(f"[gpu] solve {name} {g.name} {params}", flush=True)
+    res = run_solve(ctx, g, rho, params)
+    # boundary values from BDYGEN
+    top = res["phi_b"][1:g.jmax + 1, g.kmax, :]
+    out = res["phi_b"][g.jmax, 1:g.kmax, :]
+    assert rel_max(top, f["phi_b_top"]) < 1e-13
+    assert rel_max(out, f["phi_b_out"]) < 1e-13
+    assert abs(res["tmass"] - float(f["tmass_b"])) <= 1e-13 * abs(float(f["tmass_b"]))
+    # the potential, every cell POT3 writes (and the ones it leaves alone)
+    assert rel_max(res["phi"], f["phi"]) < 1e-12
+    # ZAXPHI diagnostics: PHICHK is a difference of nearly equal numbers
+    assert abs(res["phichk"] - float(f["phichk"])) < 1e-10
+    assert res["klocat"] == int(f["klocat"])
+    ck = res["coef"][:, 1, :, :]
+    amp_ref, amp = f["coef_k2"][..., 0], ck[..., 0]
+    assert np.abs(amp - amp_ref).max() < 1e-11
+    sig = amp_ref > 1e-8
+    assert np.abs(phase_diff(ck[..., 1], f["coef_k2"][..., 1])[sig]).max() < 1e-6
+    if "coef" in f.files:
+        c = res["coef"]
+        rc = f["coef"]
+        assert np.abs(c[..., 0] - rc[..., 0]).max() < 1e-11
+        sig = rc[..., 0] > 1e-8
+        assert np.abs(phase_diff(c[..., 1], rc[..., 1])[sig]).max() < 1e-6
+
+
+def test_untouched_cells_and_determinism(ctx):
+    g, rho, params, f = load_case("g24x16x32_spiral")
+    a = run_solve(ctx, g, rho, params, phi_init=7.25)
+    b = run_solve(ctx, g, rho, params, phi_init=7.25)
+    assert np.array_equal(a["phi"], b["phi"])
+    # POT3 leaves the K=KMAX+2 row and PHI(JMAX+2,1,L) alone (pot3.f:287-298,377)
+    assert np.all(a["phi"][:, g.kmax + 1, :] == 7.25)
+    assert np.all(a["phi"][g.jmax + 1, 0, :] == 7.25)
+
+
+def test_host_entry_points_match_device(ctx):
+    g, rho, params, f = load_case("g32_redge")
+    dev = run_solve(ctx, g, rho, params)
+    pg = chypot.PreparedGrid(ctx, g, isym=params["isym"])
+    phi = np.zeros(g.shape, order="F")
+    coef = np.zeros((g.jmax + 2, g.kmax + 2, g.lmax // 2, 2), order="F")
+    pg.bdygen_host(rho, phi, maxtrm=params["maxtrm"], redge=params["redge"])
+    d = pg.pot3_host(rho, phi, npoint=params["npoint"], coef=coef, coef_mode=chypot.COEF_FULL)
+    assert np.array_equal(phi, dev["phi"])
+    assert np.array_equal(coef, dev["coef"])
+    assert d.klocat == dev["klocat"]
+
+
+def test_config2_against_reference_and_exact(ctx):
+    f = np.load(GOLDEN / "solve_large_256x128x256.npz")
+    g = fields.unit_grid(256, 128, 256)
+    rho = fields.density_spiral_torus(g, seed=2)
+    res = run_solve(ctx, g, rho, dict(isym=2, maxtrm=10, npoint=8, redge=0.0), coef_mode=chypot.COEF_K2)
+    phi = res["phi"]
+    scale = float(f["phi_absmax"])
+    e_ref = max(rel_max(phi[:, 1, :], f["phi_k2"], scale), rel_max(phi[:, :, 0], f["phi_l1"], scale))
+    e_ext = max(rel_max(phi[:, 1, :], f["ext_phi_k2"], scale), rel_max(phi[:, :, 0], f["ext_phi_l1"], scale))
+    print(f"[gpu] 256x128x256: vs reference {e_ref:.3e}, vs 80-bit {e_ext:.3e}, "
+          f"reference vs 80-bit {float(f['ext_ref_maxdiff']):.3e}", flush=True)
+    # the reference's own FP64 error at this size is ~2.3e-12 (vs the 80-bit solve)
+    assert e_ref < 4e-12
+    assert e_ext < 1.5e-12
+
+
+@pytest.mark.parametrize("jkl,code", [((16, 6, 16), chypot.CHY_EUNSUPPORTED),
+                                      ((16, 8, 12), chypot.CHY_EUNSUPPORTED),
+                                      ((2, 8, 16), chypot.CHY_EINVAL)])
+def test_rejected_grids(ctx, jkl, code):
+    g = fields.unit_grid(*jkl)
+    with pytest.raises(chypot.ChyError) as ei:
+        chypot.PreparedGrid(ctx, g)
+    assert ei.value.code == code
+
+
+def test_bad_arguments(ctx):
+    g = fields.unit_grid(16, 8, 16)
+    pg = chypot.PreparedGrid(ctx, g)
+    rho = torch.zeros((16, 10, 18), dtype=torch.float64, device="cuda")
+    phi = torch.zeros_like(rho)
+    with pytest.raises(chypot.ChyError) as ei:
+        pg.pot3(rho, phi, npoint=0)
+    assert ei.value.code == chypot.CHY_EINVAL
+    with pytest.raises(chypot.ChyError) as ei:
+        pg.bdygen(rho, phi, maxtrm=11)
+    assert ei.value.code == chypot.CHY_EINVAL
+    with pytest.raises(chypot.ChyError) as ei:
+        pg.bdygen(rho, phi, isym=1)
+    assert ei.value.code == chypot.CHY_EINVAL
