@@ -25,14 +25,14 @@ LIB_PATH = _HERE.parent / "_lib" / "libchypot.so"
 
 CHY_OK, CHY_EINVAL, CHY_EUNSUPPORTED, CHY_ENOMEM, CHY_EDEVICE, CHY_ESTATE, CHY_ENUMERIC = range(7)
 COEF_NONE, COEF_K2, COEF_FULL = 0, 1, 2
-STAGES = ("bdygen", "fft_fwd", "zfwd", "rsolve", "zinv", "fft_inv", "ghost")
+STAGES = ("bdygen", "fft_fwd", "zfwd", "rsolve", "zinv", "fft_inv", "ghost", "fft_bdy")
 
 # every entry point declared in include/chypot.h
 EXPORTS = (
     "chy_ctx_create", "chy_ctx_destroy", "chy_ctx_set_stream", "chy_ctx_sync",
     "chy_setbdy", "chy_grid_destroy", "chy_grid_device_bytes",
     "chy_bdygen", "chy_pot3", "chy_solve", "chy_bdygen_host", "chy_pot3_host",
-    "chy_ctx_profile", "chy_ctx_stage_times",
+    "chy_ctx_profile", "chy_ctx_stage_times", "chy_ctx_graphs",
     "chy_strerror", "chy_last_device_error",
 )
 
@@ -83,6 +83,7 @@ def lib():
         L.chy_bdygen_host.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, dp, C.POINTER(Diag)]
         L.chy_pot3_host.argtypes = [vp, C.c_int, dp, dp, dp, C.c_int, C.POINTER(Diag)]
         L.chy_ctx_profile.argtypes = [vp, C.c_int]
+        L.chy_ctx_graphs.argtypes = [vp, C.c_int]
         L.chy_ctx_stage_times.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int]
         L.chy_strerror.argtypes = [C.c_int]
         L.chy_strerror.restype = C.c_char_p
@@ -126,6 +127,10 @@ class Context:
     def profile(self, enable: bool = True):
         """Bracket every solver stage with HIP events (resets the totals)."""
         _check(lib().chy_ctx_profile(self.handle, int(enable)), "chy_ctx_profile")
+
+    def graphs(self, enable: bool = True):
+        """Replay chy_solve from captured HIP graphs (default on)."""
+        _check(lib().chy_ctx_graphs(self.handle, int(enable)), "chy_ctx_graphs")
 
     def stage_times(self):
         """{stage: (total_ms, launches)} since profiling was enabled (waits)."""

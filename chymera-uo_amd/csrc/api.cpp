@@ -28,11 +28,11 @@ static int dev_rc(int hip_rc, const char *where)
 
 static void free_grid_memory(chy_grid *g)
 {
-    for (void *p : g->allocations) hipFree(p);
+    for (void *p : g->allocations) (void)hipFree(p);
     g->allocations.clear();
-    if (g->d.h_rho) hipFree(g->d.h_rho);
-    if (g->d.h_phi) hipFree(g->d.h_phi);
-    if (g->d.h_coef) hipFree(g->d.h_coef);
+    if (g->d.h_rho) (void)hipFree(g->d.h_rho);
+    if (g->d.h_phi) (void)hipFree(g->d.h_phi);
+    if (g->d.h_coef) (void)hipFree(g->d.h_coef);
     g->d.h_rho = g->d.h_phi = g->d.h_coef = nullptr;
 }
 
@@ -40,14 +40,14 @@ struct DeviceScope {
     int prev = -1;
     explicit DeviceScope(int dev)
     {
-        hipGetDevice(&prev);
-        if (prev != dev) hipSetDevice(dev);
+        (void)hipGetDevice(&prev);
+        if (prev != dev) (void)hipSetDevice(dev);
     }
     ~DeviceScope()
     {
         int cur = -1;
-        hipGetDevice(&cur);
-        if (prev >= 0 && cur != prev) hipSetDevice(prev);
+        (void)hipGetDevice(&cur);
+        if (prev >= 0 && cur != prev) (void)hipSetDevice(prev);
     }
 };
 
@@ -114,26 +114,36 @@ void Profiler::reset()
     for (int i = 0; i < CHY_NSTAGES; ++i) { ms[i] = 0.0; count[i] = 0; }
 }
 
-static int run_bdygen(chy_grid *g, int maxtrm, int isym, double redge, const double *rho, double *phi)
+static bool bdygen_args_ok(int maxtrm, int isym, double redge)
 {
-    if (maxtrm < 0 || maxtrm > kMaxTrm || std::abs(isym) != 2 || !(redge >= 0.0)) return CHY_EINVAL;
-    Profiler &p = g->ctx->prof;
-    p.begin(CHY_STAGE_BDYGEN, g->ctx->stream);
-    int rc = dev_rc(launch_bdygen(g, maxtrm, isym, redge, rho, phi, g->ctx->stream), "bdygen launch");
-    p.end(g->ctx->stream);
-    return rc;
+    return maxtrm >= 0 && maxtrm <= kMaxTrm && std::abs(isym) == 2 && redge >= 0.0;
 }
 
-static int run_pot3(chy_grid *g, int npoint, const double *rho, double *phi, double *coef, int coef_mode,
-                    int klocat_in)
+// BDYGEN alone: ring moments from a transform of the interior lanes (no
+// spectrum kept), then the radial sums and the boundary values.
+static int run_bdygen(chy_grid *g, int maxtrm, int isym, double redge, const double *rho, double *phi)
 {
+    if (!bdygen_args_ok(maxtrm, isym, redge)) return CHY_EINVAL;
     hipStream_t s = g->ctx->stream;
     Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
     if (p) p->begin(CHY_STAGE_FFT_FWD, s);
-    int rc = dev_rc(launch_fft_forward(g, rho, phi, coef, coef_mode, s), "forward FFT launch");
+    int rc = dev_rc(launch_fft_forward(g, rho, phi, nullptr, CHY_COEF_NONE, LANES_INTERIOR, true, false, s),
+                    "moment transform launch");
     if (p) p->end(s);
     if (rc) return rc;
-    rc = dev_rc(launch_mode_solve(g, s, p), "mode solve launch");
+    if (p) p->begin(CHY_STAGE_BDYGEN, s);
+    rc = dev_rc(launch_bdygen_from_moments(g, maxtrm, isym, redge, phi, false, nullptr, CHY_COEF_NONE, s),
+                "bdygen launch");
+    if (p) p->end(s);
+    return rc;
+}
+
+// the solve after the forward transform: mode solves, inverse transform, ghosts, ZAXPHI
+static int run_pot3_tail(chy_grid *g, int npoint, double *phi, int klocat_in)
+{
+    hipStream_t s = g->ctx->stream;
+    Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
+    int rc = dev_rc(launch_mode_solve(g, s, p), "mode solve launch");
     if (rc) return rc;
     if (p) p->begin(CHY_STAGE_FFT_INV, s);
     rc = dev_rc(launch_fft_inverse(g, phi, s), "inverse FFT launch");
@@ -143,6 +153,85 @@ static int run_pot3(chy_grid *g, int npoint, const double *rho, double *phi, dou
     rc = dev_rc(launch_ghosts_zaxphi(g, npoint, klocat_in, phi, s), "ghost/ZAXPHI launch");
     if (p) p->end(s);
     return rc;
+}
+
+// BDYGEN + POT3 with the density read once: the interior transform writes
+// both the spectrum and the ring moments; BDYGEN then writes the boundary
+// lanes' spectrum directly (a Fourier series of order <= MAXTRM).  For
+// LMAX/2 <= 10 the boundary values go through PHI and a transform instead.
+static int run_solve_eager(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho,
+                           double *phi, double *coef, int coef_mode, int klocat_in)
+{
+    hipStream_t s = g->ctx->stream;
+    Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
+    const bool spectral = bdygen_spectral_ok(g);
+    if (p) p->begin(CHY_STAGE_FFT_FWD, s);
+    int rc = dev_rc(launch_fft_forward(g, rho, phi, coef, coef_mode, LANES_INTERIOR, true, true, s),
+                    "forward FFT launch");
+    if (p) p->end(s);
+    if (rc) return rc;
+    if (p) p->begin(CHY_STAGE_BDYGEN, s);
+    rc = dev_rc(launch_bdygen_from_moments(g, maxtrm, isym, redge, phi, spectral, coef, coef_mode, s),
+                "bdygen launch");
+    if (p) p->end(s);
+    if (rc) return rc;
+    if (!spectral) {
+        if (p) p->begin(CHY_STAGE_FFT_BDY, s);
+        rc = dev_rc(launch_fft_forward(g, rho, phi, coef, coef_mode, LANES_BOUNDARY, false, true, s),
+                    "boundary FFT launch");
+        if (p) p->end(s);
+        if (rc) return rc;
+    }
+    return run_pot3_tail(g, npoint, phi, klocat_in);
+}
+
+// The same sequence replayed from a HIP graph (captured on first use for a
+// given set of arguments) when stage timing is off: one launch per solve.
+static int run_solve(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho,
+                     double *phi, double *coef, int coef_mode, int klocat_in)
+{
+    if (!bdygen_args_ok(maxtrm, isym, redge)) return CHY_EINVAL;
+    chy_ctx *c = g->ctx;
+    if (c->prof.on || !c->graphs)
+        return run_solve_eager(g, maxtrm, isym, redge, npoint, rho, phi, coef, coef_mode, klocat_in);
+    SolveKey key{rho, phi, coef, coef_mode, maxtrm, isym, npoint, klocat_in, redge, c->stream};
+    if (!g->graph_exec || !(g->graph_key == key)) {
+        if (g->graph_exec) {
+            (void)hipGraphExecDestroy(g->graph_exec);
+            g->graph_exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return dev_rc((int)e, "hipStreamBeginCapture");
+        int rc = run_solve_eager(g, maxtrm, isym, redge, npoint, rho, phi, coef, coef_mode, klocat_in);
+        e = hipStreamEndCapture(c->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return dev_rc((int)e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&g->graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            g->graph_exec = nullptr;
+            return dev_rc((int)e, "hipGraphInstantiate");
+        }
+        g->graph_key = key;
+    }
+    return dev_rc((int)hipGraphLaunch(g->graph_exec, c->stream), "hipGraphLaunch");
+}
+
+static int run_pot3(chy_grid *g, int npoint, const double *rho, double *phi, double *coef, int coef_mode,
+                    int klocat_in)
+{
+    hipStream_t s = g->ctx->stream;
+    Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
+    if (p) p->begin(CHY_STAGE_FFT_FWD, s);
+    int rc = dev_rc(launch_fft_forward(g, rho, phi, coef, coef_mode, LANES_ALL, false, true, s),
+                    "forward FFT launch");
+    if (p) p->end(s);
+    if (rc) return rc;
+    return run_pot3_tail(g, npoint, phi, klocat_in);
 }
 
 static int read_diag(chy_grid *g, chy_diag *diag, bool tmass, bool zax)
@@ -246,6 +335,13 @@ int chy_ctx_stage_times(chy_ctx *ctx, double *ms, int *counts, int n)
     return rc;
 }
 
+int chy_ctx_graphs(chy_ctx *ctx, int enable)
+{
+    if (!ctx) return CHY_EINVAL;
+    ctx->graphs = enable != 0;
+    return CHY_OK;
+}
+
 int chy_ctx_set_stream(chy_ctx *ctx, void *stream)
 {
     if (!ctx) return CHY_EINVAL;
@@ -282,7 +378,9 @@ int chy_grid_destroy(chy_grid *g)
 {
     if (!g) return CHY_EINVAL;
     DeviceScope ds(g->ctx->device);
-    hipStreamSynchronize(g->ctx->stream);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
+    g->graph_exec = nullptr;
     free_grid_memory(g);
     delete g;
     return CHY_OK;
@@ -322,9 +420,7 @@ int chy_solve(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const
     int rc = check_pot3_args(g, npoint, coef_mode, coef_d);
     if (rc) return rc;
     DeviceScope ds(g->ctx->device);
-    rc = run_bdygen(g, maxtrm, isym, redge, rho_d, phi_d);
-    if (rc) return rc;
-    rc = run_pot3(g, npoint, rho_d, phi_d, coef_d, coef_mode, diag ? diag->klocat : 0);
+    rc = run_solve(g, maxtrm, isym, redge, npoint, rho_d, phi_d, coef_d, coef_mode, diag ? diag->klocat : 0);
     if (rc == CHY_OK && diag) rc = read_diag(g, diag, true, true);
     return rc;
 }

@@ -1,19 +1,23 @@
 // BDYGEN (pot/boundary.f:88-908): multipole boundary potential.
 //
 // The reference loops over every ring (JJ,KK) and, for each of the 383
-// boundary points, adds the ring's 66 (l,m) moments either to MASSIN (ring
-// inside the point's spherical radius) or, scaled by r^-(2l+1), to MASSOUT
+// boundary points, adds the ring's (l,m) moments either to MASSIN (ring inside
+// the point's spherical radius) or, scaled by r^-(2l+1), to MASSOUT
 // (boundary.f:556-604): 1.64 G branchy adds at 256x128x256.  The split only
 // depends on the ring's spherical radius, so here the rings are sorted by
-// radius once per grid; then MASSIN of a boundary point is a prefix sum and
-// MASSOUT a suffix sum over the sorted rings.  Kernels:
-//   k_bdy_moments  ring moments SUMASS, CM(m), SM(m) over L (boundary.f:526-538)
+// radius once per grid; MASSIN of a boundary point is then a prefix sum and
+// MASSOUT a suffix sum over the sorted rings.
+//
+// The ring moments SUMASS, CM(m), SM(m) (boundary.f:526-538) are azimuthal
+// Fourier coefficients at the cell-centre angles; the forward transform of the
+// interior lanes writes them (fft.hip), so the density is read once per solve.
+// Kernels here:
 //   k_bdy_chunks   per chunk of <= 64 radius-sorted rings: inside- and
-//                  outside-form sums of TERM(l,m) x moment for the 36 l+m-even
+//                  outside-form sums of TERM(l,m) x moment for the l+m-even
 //                  terms (boundary.f:541-553 and :585-593)
 //   k_bdy_scan     exclusive prefix (inside) / suffix (outside) over chunks;
-//                  chunk boundaries contain every boundary point's split, so
-//                  MASSIN/MASSOUT of point jk are one entry each
+//                  chunk boundaries include every boundary point's split, so
+//                  MASSIN/MASSOUT of a point are one entry each
 //   k_bdy_eval     TRMTOT and PHI at the boundary points for all L
 //                  (boundary.f:659-830)
 //   k_bdy_ghosts   ISYM=2 ghosts (boundary.f:876-893)
@@ -25,13 +29,12 @@ namespace chy {
 namespace {
 
 constexpr int NCOL = 2 * kMaxTerms;  // (term, cos|sin) columns
+constexpr int CHUNK_GROUPS = 7;      // ring groups per chunk block (7 x 36 <= 256 threads)
 
 struct BdyArgs {
-    const double *rho;
     double *phi;
-    const double *dv;
     const double *cosm, *sinm;
-    double *mom;
+    const double *mom;
     const int *ring_of_pos;
     const double *tin, *tout, *rsph;
     const int *chunk_start;
@@ -42,61 +45,50 @@ struct BdyArgs {
     const int *jpos, *kpos;
     double *dscal;
     int J, K, L, JS, KS, nring, nchunk, jkmax, nt, isym;
-    double redge, grav;
-    int term_l[kMaxTerms], term_m[kMaxTerms];  // (l,m) of each term, l-major
+    double redge, grav, pi;
 };
 
-__global__ __launch_bounds__(256) void k_bdy_moments(BdyArgs a)
+// chunk[c][0][2t+s]: sum over the chunk's rings of TERM_t * moment_s (inside
+// form), chunk[c][1][..]: the same with TERM_t r^-(2l+1) (outside form);
+// s=0 cosine (SUMASS for m=0), s=1 sine.
+__global__ __launch_bounds__(256) void k_bdy_chunks(BdyArgs a)
 {
-    const int ir = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ir >= a.nring) return;
-    const int jj = 2 + ir % (a.J - 1), kk = 2 + ir / (a.J - 1);
-    const size_t plane = (size_t)a.JS * a.KS;
-    const double *rp = a.rho + (size_t)(jj - 1) + (size_t)a.JS * (kk - 1);
-    const double dv = a.dv[ir];
-    double s = 0.0, cmv[10], smv[10];
-#pragma unroll
-    for (int m = 0; m < 10; ++m) { cmv[m] = 0.0; smv[m] = 0.0; }
-    for (int l = 0; l < a.L; ++l) {
-        const double h = rp[plane * l] * dv;
-        s += h;
-#pragma unroll
-        for (int m = 0; m < 10; ++m) {
-            cmv[m] = fma(a.cosm[l * 10 + m], h, cmv[m]);
-            smv[m] = fma(a.sinm[l * 10 + m], h, smv[m]);
-        }
-    }
-    double *o = a.mom + (size_t)ir * kMoments;
-    o[0] = s;
-#pragma unroll
-    for (int m = 0; m < 10; ++m) {
-        o[1 + m] = cmv[m];
-        o[11 + m] = smv[m];
-    }
-}
-
-// chunk[c][0][col]: sum TERM*moment (inside form), chunk[c][1][col]: outside form
-__global__ __launch_bounds__(128) void k_bdy_chunks(BdyArgs a)
-{
+    __shared__ double part[CHUNK_GROUPS][kMaxTerms][4];
     const int c = blockIdx.x;
-    const int col = threadIdx.x;
-    if (col >= NCOL) return;
-    const int t = col >> 1, cs = col & 1;
-    double ai = 0.0, ao = 0.0;
-    if (t < a.nt) {
-        const int m = a.term_m[t];
-        const int midx = m == 0 ? (cs ? -1 : 0) : (cs ? 10 + m : m);
-        const int p0 = a.chunk_start[c], p1 = a.chunk_start[c + 1];
-        for (int pos = p0; pos < p1; ++pos) {
-            if (a.redge > 0.0 && a.rsph[pos] > a.redge) continue;  // boundary.f:379
-            const int ir = a.ring_of_pos[pos];
-            const double mv = midx < 0 ? 0.0 : a.mom[(size_t)ir * kMoments + midx];
-            ai = fma(a.tin[(size_t)pos * kMaxTerms + t], mv, ai);
-            ao = fma(a.tout[(size_t)pos * kMaxTerms + t], mv, ao);
+    const int tid = threadIdx.x;
+    const int p0 = a.chunk_start[c], p1 = a.chunk_start[c + 1];
+    if (tid < CHUNK_GROUPS * kMaxTerms) {
+        const int t = tid % kMaxTerms, grp = tid / kMaxTerms;
+        double ic = 0.0, is = 0.0, oc = 0.0, os = 0.0;
+        if (t < a.nt) {
+            const int m = kTermM[t];
+            for (int pos = p0 + grp; pos < p1; pos += CHUNK_GROUPS) {
+                if (a.redge > 0.0 && a.rsph[pos] > a.redge) continue;  // boundary.f:379
+                const double *mo = a.mom + (size_t)a.ring_of_pos[pos] * kMoments;
+                const double mc = mo[m];                     // SUMASS (m=0) or CM(m)
+                const double ms = m == 0 ? 0.0 : mo[10 + m];  // SM(m)
+                const double ti = a.tin[(size_t)pos * kMaxTerms + t];
+                const double to = a.tout[(size_t)pos * kMaxTerms + t];
+                ic = fma(ti, mc, ic);
+                is = fma(ti, ms, is);
+                oc = fma(to, mc, oc);
+                os = fma(to, ms, os);
+            }
         }
+        part[grp][t][0] = ic;
+        part[grp][t][1] = is;
+        part[grp][t][2] = oc;
+        part[grp][t][3] = os;
     }
-    a.chunk[((size_t)c * 2 + 0) * NCOL + col] = ai;
-    a.chunk[((size_t)c * 2 + 1) * NCOL + col] = ao;
+    __syncthreads();
+    if (tid < kMaxTerms * 4) {
+        const int t = tid / 4, v = tid % 4;
+        double s = 0.0;
+#pragma unroll
+        for (int grp = 0; grp < CHUNK_GROUPS; ++grp) s += part[grp][t][v];
+        const int comp = v >> 1, cs = v & 1;
+        a.chunk[((size_t)c * 2 + comp) * NCOL + 2 * t + cs] = s;
+    }
 }
 
 // massio[c][0][col] = sum_{c'<c} chunk[c'][0][col]     (c = 0..nchunk)
@@ -125,22 +117,15 @@ __global__ __launch_bounds__(256) void k_bdy_scan(BdyArgs a)
             __syncthreads();
         }
         const double incl = carry + buf[tid];
-        const double excl = incl - v;  // not used for sums; recomputed below
-        (void)excl;
         if (i < n) {
-            if (comp == 0) {
-                // exclusive prefix up to c=i is the inclusive prefix up to i-1
-                a.massio[((size_t)(i + 1) * 2 + 0) * NCOL + col] = incl;
-            } else {
-                const int c = n - 1 - i;
-                a.massio[((size_t)c * 2 + 1) * NCOL + col] = incl;
-            }
+            if (comp == 0) a.massio[((size_t)(i + 1) * 2 + 0) * NCOL + col] = incl;  // prefix of c' <= i
+            else a.massio[((size_t)(n - 1 - i) * 2 + 1) * NCOL + col] = incl;      // suffix from c = n-1-i
         }
         carry += buf[255];
         __syncthreads();
     }
     if (tid == 0) {
-        if (comp == 0) a.massio[(size_t)0 * 2 * NCOL + col] = 0.0;
+        if (comp == 0) a.massio[col] = 0.0;
         else a.massio[((size_t)n * 2 + 1) * NCOL + col] = 0.0;
         if (comp == 0 && col == 0) a.dscal[0] = carry;  // TMASS = sum of MRING(1,1)
     }
@@ -149,40 +134,149 @@ __global__ __launch_bounds__(256) void k_bdy_scan(BdyArgs a)
 __global__ __launch_bounds__(256) void k_bdy_eval(BdyArgs a)
 {
     __shared__ double t1[kMaxTerms], t2[kMaxTerms];
-    __shared__ int tm[kMaxTerms];
     const int jk = blockIdx.x;
     const int tid = threadIdx.x;
-    const double rs = a.rb[jk];
-    if (tid < a.nt) {
-        const int l = a.term_l[tid];
-        double rout = 1.0;
-        for (int i = 0; i < l; ++i) rout *= rs;
-        const double rin = 1.0 / (rout * rs);  // 1/RSPHER**(L+1)
-        const double term = a.tb[(size_t)jk * kMaxTerms + tid];
-        const double b = term * rin, h = term * rout;
-        const int c = a.isym > 0 ? a.jk_cut[jk] : a.nchunk;
-        const double *mi = a.massio + ((size_t)c * 2 + 0) * NCOL;
-        const double *mo = a.massio + ((size_t)c * 2 + 1) * NCOL;
-        if (a.isym > 0) {
-            t1[tid] = b * mi[2 * tid] + h * mo[2 * tid];
-            t2[tid] = b * mi[2 * tid + 1] + h * mo[2 * tid + 1];
-        } else {
-            t1[tid] = b * mi[2 * tid];
-            t2[tid] = b * mi[2 * tid + 1];
+    if (tid < kMaxTerms) {
+        double v1 = 0.0, v2 = 0.0;
+        if (tid < a.nt) {
+            const double rs = a.rb[jk];
+            const int l = kTermL[tid];
+            double rout = 1.0;
+            for (int i = 0; i < l; ++i) rout *= rs;
+            const double rin = 1.0 / (rout * rs);  // 1/RSPHER**(L+1)
+            const double term = a.tb[(size_t)jk * kMaxTerms + tid];
+            const double b = term * rin, h = term * rout;
+            const int c = a.isym > 0 ? a.jk_cut[jk] : a.nchunk;
+            const double *mi = a.massio + ((size_t)c * 2 + 0) * NCOL;
+            const double *mo = a.massio + ((size_t)c * 2 + 1) * NCOL;
+            if (a.isym > 0) {
+                v1 = b * mi[2 * tid] + h * mo[2 * tid];
+                v2 = b * mi[2 * tid + 1] + h * mo[2 * tid + 1];
+            } else {  // all mass inside (boundary.f:795-805)
+                v1 = b * mi[2 * tid];
+                v2 = b * mi[2 * tid + 1];
+            }
         }
-        tm[tid] = a.term_m[tid];
+        t1[tid] = v1;
+        t2[tid] = v2;
     }
     __syncthreads();
     const size_t plane = (size_t)a.JS * a.KS;
     const size_t pbase = (size_t)(a.jpos[jk] - 1) + (size_t)a.JS * (a.kpos[jk] - 1);
     for (int l = tid; l < a.L; l += blockDim.x) {
+        double cr[10], sr[10];
+#pragma unroll
+        for (int m = 0; m < 10; ++m) {
+            cr[m] = a.cosm[l * 10 + m];
+            sr[m] = a.sinm[l * 10 + m];
+        }
+        // SUM over (l,m) in the reference's order (boundary.f:819-828); terms
+        // beyond MAXTRM hold 0 and leave the sum unchanged
         double sum = 0.0;
-        for (int t = 0; t < a.nt; ++t) {
-            const int m = tm[t];
-            if (m == 0) sum += t1[t];
-            else sum += 2.0 * (a.cosm[l * 10 + m - 1] * t1[t] + a.sinm[l * 10 + m - 1] * t2[t]);
+#pragma unroll
+        for (int t = 0; t < kMaxTerms; ++t) {
+            if (kTermM[t] == 0) sum += t1[t];
+            else sum += 2.0 * (cr[kTermM[t] - 1] * t1[t] + sr[kTermM[t] - 1] * t2[t]);
         }
         a.phi[pbase + plane * l] = -a.grav * sum;
+    }
+}
+
+// The boundary potential of BDYGEN is a Fourier series in psi with orders
+// m <= MAXTRM (boundary.f:814-829); its packed azimuthal spectrum (what POT3's
+// forward transform of the boundary lanes yields, pot3.f:92-140) follows in
+// closed form.  With C_m = sum_{t: m(t)=m} TRMTOT(1,t), S_m likewise for
+// TRMTOT(2,t), and psi_n = (n+1/2) DTHETA:
+//   a_0 = -2 G C_0
+//   a_m = -2 G ( C_m cos(m DTHETA/2) + S_m sin(m DTHETA/2))     1 <= m <= MAXTRM
+//   b_m = -2 G (-C_m sin(m DTHETA/2) + S_m cos(m DTHETA/2))
+// and every other coefficient is 0.  Valid for LMAX/2 > MAXTRM.  Writes the
+// spectrum of the boundary lane and its COEF entries (pot3.f:105-132).
+__global__ __launch_bounds__(64) void k_bdy_spec(BdyArgs a, double *spec, const double2 *twh, double *coef,
+                                                 int coef_mode)
+{
+    __shared__ double t1[kMaxTerms], t2[kMaxTerms];
+    __shared__ double cmv[kMaxTrm + 1], smv[kMaxTrm + 1];
+    const int jk = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (tid < kMaxTerms) {
+        double v1 = 0.0, v2 = 0.0;
+        if (tid < a.nt) {
+            const double rs = a.rb[jk];
+            const int l = kTermL[tid];
+            double rout = 1.0;
+            for (int i = 0; i < l; ++i) rout *= rs;
+            const double rin = 1.0 / (rout * rs);
+            const double term = a.tb[(size_t)jk * kMaxTerms + tid];
+            const double b = term * rin, h = term * rout;
+            const int c = a.isym > 0 ? a.jk_cut[jk] : a.nchunk;
+            const double *mi = a.massio + ((size_t)c * 2 + 0) * NCOL;
+            const double *mo = a.massio + ((size_t)c * 2 + 1) * NCOL;
+            if (a.isym > 0) {
+                v1 = b * mi[2 * tid] + h * mo[2 * tid];
+                v2 = b * mi[2 * tid + 1] + h * mo[2 * tid + 1];
+            } else {
+                v1 = b * mi[2 * tid];
+                v2 = b * mi[2 * tid + 1];
+            }
+        }
+        t1[tid] = v1;
+        t2[tid] = v2;
+    }
+    __syncthreads();
+    if (tid <= kMaxTrm) {
+        double c = 0.0, s = 0.0;
+#pragma unroll
+        for (int t = 0; t < kMaxTerms; ++t)
+            if (kTermM[t] == tid) { c += t1[t]; s += t2[t]; }
+        cmv[tid] = c;
+        smv[tid] = s;
+    }
+    __syncthreads();
+    const int N = a.L / 2;
+    const int j = a.jpos[jk] - 2, k = a.kpos[jk] - 2;
+    const size_t JK = (size_t)a.J * a.K, sidx = (size_t)k * a.J + j;
+    const double g2 = -2.0 * a.grav;
+    const double a0 = g2 * cmv[0];
+    for (int q = tid; q < a.L; q += blockDim.x) {
+        double v = 0.0;
+        if (q == 0) {
+            v = a0;
+        } else if (q <= kMaxTrm) {
+            v = g2 * (cmv[q] * twh[q].x + smv[q] * twh[q].y);
+        } else if (q > N && q - N <= kMaxTrm) {
+            const int m = q - N;
+            v = g2 * (-cmv[m] * twh[m].y + smv[m] * twh[m].x);
+        }
+        spec[(size_t)q * JK + sidx] = v;
+    }
+    if (coef && (coef_mode == CHY_COEF_FULL || (coef_mode == CHY_COEF_K2 && k == 0))) {
+        // raw REALTR coefficients are the packed ones times L (SF1 = 1/L)
+        const size_t plane = (size_t)a.JS * a.KS;
+        const size_t base = (size_t)(j + 1) + (size_t)a.JS * (size_t)(k + 1);
+        const double A0 = a0 * a.L, x1 = fabs(A0), pi = a.pi;
+        for (int m = 1 + tid; m <= N; m += blockDim.x) {
+            double A = 0.0, B = 0.0;
+            if (m <= kMaxTrm && m < N) {
+                A = a.L * g2 * (cmv[m] * twh[m].x + smv[m] * twh[m].y);
+                B = a.L * g2 * (-cmv[m] * twh[m].y + smv[m] * twh[m].x);
+            }
+            double *c1 = coef + base + plane * (size_t)(m - 1);
+            double *c2 = c1 + plane * (size_t)N;
+            if (m < N) {
+                if (x1 != 0.0) *c1 = 2.0 * sqrt(A * A + B * B) / x1;
+                double philag;
+                if (A > 0.0) philag = atan(B / A);
+                else if (A < 0.0) philag = atan(B / A) + pi;
+                else if (B > 0.0) philag = 0.5 * pi;
+                else if (B < 0.0) philag = -0.5 * pi;
+                else philag = 4.0 * pi;
+                *c2 = philag * 180.0 / pi;
+            } else {
+                if (x1 != 0.0) *c1 = fabs(A) / x1;
+                *c2 = A0;
+            }
+        }
     }
 }
 
@@ -202,16 +296,16 @@ __global__ __launch_bounds__(256) void k_bdy_ghosts(BdyArgs a)
 
 }  // namespace
 
-int launch_bdygen(const chy_grid *g, int maxtrm, int isym, double redge, const double *rho,
-                  double *phi, hipStream_t s)
+bool bdygen_spectral_ok(const chy_grid *g) { return g->L / 2 > kMaxTrm; }
+
+int launch_bdygen_from_moments(const chy_grid *g, int maxtrm, int isym, double redge, double *phi,
+                               bool spectral, double *coef, int coef_mode, hipStream_t s)
 {
-    int n = 0;
-    const Term *tt = terms_table(&n);
     int nt = 0;
-    while (nt < n && tt[nt].l <= maxtrm) ++nt;
+    while (nt < kMaxTerms && kTermL[nt] <= maxtrm) ++nt;
     BdyArgs a{};
-    a.rho = rho; a.phi = phi;
-    a.dv = g->d.dv; a.cosm = g->d.cosm; a.sinm = g->d.sinm;
+    a.phi = phi;
+    a.cosm = g->d.cosm; a.sinm = g->d.sinm;
     a.mom = g->d.mom; a.ring_of_pos = g->d.ring_of_pos;
     a.tin = g->d.tin; a.tout = g->d.tout; a.rsph = g->d.rsph;
     a.chunk_start = g->d.chunk_start; a.chunk = g->d.chunk; a.massio = g->d.massio;
@@ -219,13 +313,16 @@ int launch_bdygen(const chy_grid *g, int maxtrm, int isym, double redge, const d
     a.jpos = g->d.jpos; a.kpos = g->d.kpos; a.dscal = g->d.dscal;
     a.J = g->J; a.K = g->K; a.L = g->L; a.JS = g->J + 2; a.KS = g->K + 2;
     a.nring = g->nring; a.nchunk = g->nchunk; a.jkmax = g->jkmax; a.nt = nt; a.isym = isym;
-    a.redge = redge; a.grav = g->grav;
-    for (int i = 0; i < kMaxTerms; ++i) { a.term_l[i] = tt[i].l; a.term_m[i] = tt[i].m; }
-    hipLaunchKernelGGL(k_bdy_moments, dim3((g->nring + 255) / 256), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_bdy_chunks, dim3(g->nchunk), dim3(128), 0, s, a);
+    a.redge = redge; a.grav = g->grav; a.pi = g->pi;
+    hipLaunchKernelGGL(k_bdy_chunks, dim3(g->nchunk), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_bdy_scan, dim3(2 * NCOL), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_bdy_eval, dim3(g->jkmax), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_bdy_ghosts, dim3((g->L + 255) / 256), dim3(256), 0, s, a);
+    if (spectral) {
+        hipLaunchKernelGGL(k_bdy_spec, dim3(g->jkmax), dim3(64), 0, s, a, g->d.spec, g->d.twh,
+                           coef_mode == CHY_COEF_NONE ? nullptr : coef, coef_mode);
+    } else {
+        hipLaunchKernelGGL(k_bdy_eval, dim3(g->jkmax), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_bdy_ghosts, dim3((g->L + 255) / 256), dim3(256), 0, s, a);
+    }
     return (int)hipGetLastError();
 }
 

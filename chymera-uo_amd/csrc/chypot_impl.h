@@ -20,6 +20,12 @@ constexpr int kMaxTrm = 10;
 constexpr int kMaxTerms = 36;
 // Ring moments per ring: SUMASS, CM(1..10), SM(1..10) (pot/boundary.f:526-538).
 constexpr int kMoments = 21;
+// (l, m) of the terms, l-major with m ascending (the order TERM(I) is summed
+// in, boundary.f:821-828); terms_table() carries the same order.
+constexpr int kTermL[kMaxTerms] = {0, 1, 2, 2, 3, 3, 4, 4, 4, 5, 5, 5, 6, 6, 6, 6, 7, 7,
+                                   7, 7, 8, 8, 8, 8, 8, 9, 9, 9, 9, 9, 10, 10, 10, 10, 10, 10};
+constexpr int kTermM[kMaxTerms] = {0, 1, 0, 2, 1, 3, 0, 2, 4, 1, 3, 5, 0, 2, 4, 6, 1, 3,
+                                   5, 7, 0, 2, 4, 6, 8, 1, 3, 5, 7, 9, 0, 2, 4, 6, 8, 10};
 // Rings per boundary chunk (the unit of the deterministic radial reduction).
 constexpr int kChunkRings = 64;
 
@@ -36,9 +42,10 @@ struct DeviceGrid {
     // FFT
     double2 *twL = nullptr;      // (cos, sin)(2 pi e / L), e = 0..L-1
     double2 *tw64 = nullptr;     // (cos, sin)(2 pi e / 64), e = 0..63 (in-register DFTs)
+    double2 *twh = nullptr;      // (cos, sin)(pi m / L), m = 0..10 (cell-centre phase of the ring moments)
     // z eigenbasis (BLKTRI's m-independent z operator, diagonalised)
     double *wf = nullptr;        // [NZ][PP]   Y -> eigen coordinates:  Yh[p] = sum_k wf[k][p] Y[k]
-    double *wi = nullptr;        // [NZ][PP]   eigen coordinates -> X:  X[k] = sum_p wi[k][p] Z[p]
+    double *wi = nullptr;        // [NZ][PP]   eigen coordinates -> X:  X[k] = sum_p wi[p][k] Z[p]
     double *lam = nullptr;       // [PP]       eigenvalues (padded entries 1)
     // r operator (pot3.f:172-194,226-230): AM(1)=0, CM(M)=0, BM per azimuthal order m
     double *am = nullptr;        // [M]
@@ -93,8 +100,26 @@ struct chy_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
+    bool graphs = true;  // replay chy_solve from captured HIP graphs
     chy::Profiler prof;
 };
+
+namespace chy {
+// Arguments a captured solve graph was recorded with.
+struct SolveKey {
+    const void *rho;
+    void *phi, *coef;
+    int coef_mode, maxtrm, isym, npoint, klocat_in;
+    double redge;
+    hipStream_t stream;
+    bool operator==(const SolveKey &o) const
+    {
+        return rho == o.rho && phi == o.phi && coef == o.coef && coef_mode == o.coef_mode &&
+               maxtrm == o.maxtrm && isym == o.isym && npoint == o.npoint && klocat_in == o.klocat_in &&
+               redge == o.redge && stream == o.stream;
+    }
+};
+}  // namespace chy
 
 struct chy_grid {
     chy_ctx *ctx = nullptr;
@@ -107,10 +132,13 @@ struct chy_grid {
     double dtheta = 0, pi = 0, grav = 0;
     double cmmax = 0, cnmax = 0, ammin = 0, pig4 = 0;
     int nring = 0, nchunk = 0, jkmax = 0;
+    int ncu = 1;                 // compute units of the device (persistent launches)
     std::vector<double> r, z, rhf, zhf;
     size_t device_bytes = 0;
     std::vector<void *> allocations;
     chy::DeviceGrid d;
+    hipGraphExec_t graph_exec = nullptr;  // last captured chy_solve
+    chy::SolveKey graph_key{};
 };
 
 namespace chy {
@@ -120,14 +148,25 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym);
 int symmetric_tridiag_eigen(int n, std::vector<long double> &d, std::vector<long double> &e,
                             std::vector<long double> &q);
 
+// Which (J,K) lanes a forward transform covers: all J=2..JMAX+1 x K=2..KMAX+1,
+// the interior J<=JMAX, K<=KMAX (density only), or the boundary row/column
+// (BDYGEN's values).
+enum { LANES_ALL = 0, LANES_INTERIOR = 1, LANES_BOUNDARY = 2 };
+
 // kernel launchers (one .hip file each); all return hipError_t as int
+// Forward azimuthal transform; optionally writes BDYGEN's ring moments of the
+// interior lanes (d.mom) and/or the spectrum (d.spec, with COEF).
 int launch_fft_forward(const chy_grid *g, const double *rho, const double *phi, double *coef,
-                       int coef_mode, hipStream_t s);
+                       int coef_mode, int lanes, bool moments, bool spectrum, hipStream_t s);
 int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s);
 int launch_mode_solve(const chy_grid *g, hipStream_t s, Profiler *prof);
 int launch_ghosts_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s);
-int launch_bdygen(const chy_grid *g, int maxtrm, int isym, double redge, const double *rho,
-                  double *phi, hipStream_t s);
+// BDYGEN from the ring moments in d.mom: radial split sums, then either the
+// boundary values and ghosts in PHI, or (spectral) the boundary lanes'
+// azimuthal spectrum written straight into d.spec with their COEF entries.
+int launch_bdygen_from_moments(const chy_grid *g, int maxtrm, int isym, double redge, double *phi,
+                               bool spectral, double *coef, int coef_mode, hipStream_t s);
+bool bdygen_spectral_ok(const chy_grid *g);
 
 void set_device_error(hipError_t e, const char *where);
 

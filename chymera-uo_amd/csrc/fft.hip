@@ -78,28 +78,52 @@ struct FftArgs {
     const double *rho;    // RHO (forward input on the interior)
     const double *phi;    // PHI (forward input on the boundary lanes)
     double *phi_out;      // PHI (inverse output)
-    double *spec;         // [L][K][J]
+    double *spec;         // [L][K][J], or null (forward: no spectrum written)
     double *coef;         // COEF(JMAX+2,KMAX+2,LMAX/2,2) or null
+    double *mom;          // ring moments [nring][21] of interior lanes, or null
+    const double *dv;     // DV per ring (boundary.f:516)
     const double2 *twL;   // exp(2 pi i e/L)
     const double2 *tw64;  // exp(2 pi i e/64)
+    const double2 *twh;   // exp(pi i m/L), m = 0..10
     int J, K, L, JS, KS;
+    int lanes;            // LANES_ALL / LANES_INTERIOR / LANES_BOUNDARY
     int nlanes;
     int coef_mode;
     double sf1, pi;
 };
 
+// lane index -> (j,k), j = J-2 (0..JMAX-1), k = K-2 (0..KMAX-1)
+__device__ __forceinline__ void lane_jk(const FftArgs &a, long lg, int &j, int &k)
+{
+    if (a.lanes == LANES_ALL) {
+        j = (int)(lg % a.J);
+        k = (int)(lg / a.J);
+    } else if (a.lanes == LANES_INTERIOR) {
+        j = (int)(lg % (a.J - 1));
+        k = (int)(lg / (a.J - 1));
+    } else if (lg < a.J) {  // top row K=KMAX+1, J=2..JMAX+1
+        j = (int)lg;
+        k = a.K - 1;
+    } else {                // outer column J=JMAX+1, K=2..KMAX
+        j = a.J - 1;
+        k = (int)(lg - a.J);
+    }
+}
+
 template <int N1, int N2, int NL>
 __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
 {
     constexpr int N = N1 * N2;
+    constexpr int L = 2 * N;
     __shared__ double2 lds[N * NL];
+    __shared__ double2 tw[L];  // exp(2 pi i e/L), staged from a.twL
     const int tid = threadIdx.x;
     const int lane = tid % NL;
     const int sub = tid / NL;
     const long lg = (long)blockIdx.x * NL + lane;
     const bool active = lg < a.nlanes;
-    const int j = active ? (int)(lg % a.J) : 0;
-    const int k = active ? (int)(lg / a.J) : 0;
+    int j = 0, k = 0;
+    lane_jk(a, active ? lg : 0, j, k);  // inactive lanes read lane 0's data and store nothing
     // interior lanes J<=JMAX, K<=KMAX carry the density (pot3.f:71-81)
     const double *src = (j <= a.J - 2 && k <= a.K - 2) ? a.rho : a.phi;
     const size_t plane = (size_t)a.JS * a.KS;
@@ -110,17 +134,14 @@ __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
         const int n = N2 * n1 + sub;
-        double x0 = 0.0, x1 = 0.0;
-        if (active) {
-            x0 = src[base + plane * (size_t)(2 * n)];
-            x1 = src[base + plane * (size_t)(2 * n + 1)];
-        }
-        v[n1] = make_double2(x0, x1);
+        v[n1] = make_double2(src[base + plane * (size_t)(2 * n)], src[base + plane * (size_t)(2 * n + 1)]);
     }
+    for (int e = tid; e < L; e += N2 * NL) tw[e] = a.twL[e];
+    __syncthreads();
     dft_reg<N1, +1>(v, a.tw64);
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) {
-        if (k1 > 0 && sub > 0) v[k1] = cmul(v[k1], a.twL[(2 * sub * k1) % a.L]);
+        if (k1 > 0 && sub > 0) v[k1] = cmul(v[k1], tw[(2 * sub * k1) % L]);
         lds[(k1 * N2 + sub) * NL + lane] = v[k1];
     }
     __syncthreads();
@@ -153,10 +174,31 @@ __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
         double2 e = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
         double2 d = csub(zm, zc);
         double2 o = make_double2(0.5 * d.y, -0.5 * d.x);
-        return cadd(e, cmul(a.twL[m], o));
+        return cadd(e, cmul(tw[m], o));
     };
     const size_t JK = (size_t)a.J * a.K;
     const size_t sidx = (size_t)k * a.J + j;
+    if (a.mom && j <= a.J - 2 && k <= a.K - 2) {
+        // ring moments of BDYGEN (boundary.f:526-538) at the cell-centre angles
+        // psi_n = (n+1/2) DTHETA:  SUMASS = DV X_0,  CM(m) + i SM(m) = DV e^{i pi m/L} X_m
+        const int ir = j + (a.J - 1) * k;
+        const double dv = a.dv[ir];
+        for (int mm = sub; mm <= kMaxTrm; mm += N2) {
+            const int r = mm % a.L;
+            double2 X = r <= N ? xm(r) : xm(a.L - r);
+            if (r > N) X.y = -X.y;
+            const double2 w = a.twh[mm];
+            const double re = X.x * w.x - X.y * w.y, im = X.y * w.x + X.x * w.y;
+            double *o = a.mom + (size_t)ir * kMoments;
+            if (mm == 0) {
+                o[0] = dv * re;
+            } else {
+                o[mm] = dv * re;
+                o[10 + mm] = dv * im;
+            }
+        }
+    }
+    if (!a.spec) return;
     const bool want_coef = a.coef && (a.coef_mode == CHY_COEF_FULL || (a.coef_mode == CHY_COEF_K2 && k == 0));
     double A0 = 0.0;
     if (want_coef) A0 = 2.0 * xm(0).x;
@@ -191,28 +233,39 @@ template <int N1, int N2, int NL>
 __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
 {
     constexpr int N = N1 * N2;
+    constexpr int L = 2 * N;
+    constexpr int NM = N / N2 + 1;  // m = sub + N2 t, t < NM, covers 0..N
     __shared__ double2 lds[N * NL];
+    __shared__ double2 tw[L];  // exp(2 pi i e/L), staged from a.twL
     const int tid = threadIdx.x;
     const int lane = tid % NL;
     const int sub = tid / NL;
     const long lg = (long)blockIdx.x * NL + lane;
     const bool active = lg < a.nlanes;
-    const int j = active ? (int)(lg % a.J) : 0;
-    const int k = active ? (int)(lg / a.J) : 0;
+    int j = 0, k = 0;
+    lane_jk(a, active ? lg : 0, j, k);  // inactive lanes read lane 0's data and store nothing
     const size_t JK = (size_t)a.J * a.K;
     const size_t sidx = (size_t)k * a.J + j;
 
     // phase 0: C_0 = a_0/2, C_N = a_N/2 (both real, packed in slot 0),
-    //          C_m = (a_m + i b_m)/2, m = 1..N-1
-    for (int m = sub; m <= N; m += N2) {
-        double av = 0.0, bv = 0.0;
-        if (active) {
-            av = a.spec[(size_t)m * JK + sidx];
-            if (m >= 1 && m <= N - 1) bv = a.spec[(size_t)(N + m) * JK + sidx];
-        }
-        if (m == 0) lds[lane].x = 0.5 * av;
-        else if (m == N) lds[lane].y = 0.5 * av;
-        else lds[m * NL + lane] = make_double2(0.5 * av, 0.5 * bv);
+    //          C_m = (a_m + i b_m)/2, m = 1..N-1.  Every load is issued
+    //          before any is used.
+    double av[NM], bv[NM];
+#pragma unroll
+    for (int t = 0; t < NM; ++t) {
+        const int m = sub + N2 * t;
+        const int ma = m <= N ? m : N;
+        const int mb = (m >= 1 && m <= N - 1) ? N + m : 0;
+        av[t] = a.spec[(size_t)ma * JK + sidx];
+        bv[t] = a.spec[(size_t)mb * JK + sidx];
+    }
+    for (int e = tid; e < L; e += N2 * NL) tw[e] = a.twL[e];
+#pragma unroll
+    for (int t = 0; t < NM; ++t) {
+        const int m = sub + N2 * t;
+        if (m == 0) lds[lane].x = 0.5 * av[t];
+        else if (m == N) lds[lane].y = 0.5 * av[t];
+        else if (m < N) lds[m * NL + lane] = make_double2(0.5 * av[t], 0.5 * bv[t]);
     }
     __syncthreads();
     // phase 0b: Z_k = (C_k + conj C_{N-k}) + i e^{-2 pi i k/L} (C_k - conj C_{N-k})
@@ -225,7 +278,7 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
             const double2 ck = lds[kk * NL + lane], cn = lds[kn * NL + lane];
             const double2 U = make_double2(ck.x + cn.x, ck.y - cn.y);
             const double2 V = make_double2(ck.x - cn.x, ck.y + cn.y);
-            double2 w = a.twL[kk];
+            double2 w = tw[kk];
             w.y = -w.y;
             const double2 wv = cmul(w, V);
             lds[kk * NL + lane] = make_double2(U.x - wv.y, U.y + wv.x);
@@ -242,7 +295,7 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
 #pragma unroll
     for (int k1 = 1; k1 < N1; ++k1) {
         if (sub > 0) {
-            double2 w = a.twL[(2 * sub * k1) % a.L];
+            double2 w = tw[(2 * sub * k1) % L];
             w.y = -w.y;
             v[k1] = cmul(v[k1], w);
         }
@@ -303,15 +356,19 @@ int dispatch(int N, const FftArgs &a, hipStream_t s)
     }
 }
 
-FftArgs make_args(const chy_grid *g)
+FftArgs make_args(const chy_grid *g, int lanes)
 {
     FftArgs a{};
     a.spec = g->d.spec;
     a.twL = g->d.twL;
     a.tw64 = g->d.tw64;
+    a.twh = g->d.twh;
+    a.dv = g->d.dv;
     a.J = g->J; a.K = g->K; a.L = g->L;
     a.JS = g->J + 2; a.KS = g->K + 2;
-    a.nlanes = g->J * g->K;
+    a.lanes = lanes;
+    a.nlanes = lanes == LANES_ALL ? g->J * g->K
+             : lanes == LANES_INTERIOR ? (g->J - 1) * (g->K - 1) : g->J + g->K - 1;
     a.sf1 = 0.5 / (double)g->N;
     a.pi = g->pi;
     return a;
@@ -320,19 +377,21 @@ FftArgs make_args(const chy_grid *g)
 }  // namespace
 
 int launch_fft_forward(const chy_grid *g, const double *rho, const double *phi, double *coef,
-                       int coef_mode, hipStream_t s)
+                       int coef_mode, int lanes, bool moments, bool spectrum, hipStream_t s)
 {
-    FftArgs a = make_args(g);
+    FftArgs a = make_args(g, lanes);
     a.rho = rho;
     a.phi = phi;
-    a.coef = coef_mode == CHY_COEF_NONE ? nullptr : coef;
+    a.coef = (coef_mode == CHY_COEF_NONE || !spectrum) ? nullptr : coef;
     a.coef_mode = coef_mode;
+    a.mom = moments ? g->d.mom : nullptr;
+    if (!spectrum) a.spec = nullptr;
     return dispatch<true>(g->N, a, s);
 }
 
 int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s)
 {
-    FftArgs a = make_args(g);
+    FftArgs a = make_args(g, LANES_ALL);
     a.phi_out = phi;
     return dispatch<false>(g->N, a, s);
 }

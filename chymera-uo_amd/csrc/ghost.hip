@@ -131,17 +131,25 @@ __global__ __launch_bounds__(256) void k_midplane(GhostArgs a)
     a.phi[(size_t)(jf - 1) + plane * l] = v;
 }
 
-__global__ void k_phichk(GhostArgs a)
+// PHICHK = PHAC(K) of largest magnitude, KLOCAT its K; the first K wins ties and
+// KLOCAT keeps its old value when every PHAC is 0 (pot3.f:391-399)
+__global__ __launch_bounds__(256) void k_phichk(GhostArgs a)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double err = 0.0;
-    double kl = (double)a.klocat_in;
-    for (int K = 2; K <= a.K + 1; ++K) {
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    double best = 0.0;
+    int bk = 0x7fffffff;
+    for (int K = 2 + threadIdx.x; K <= a.K + 1; K += blockDim.x) {
         const double v = a.dscal[4 + K];
-        if (fabs(v) > fabs(err)) { err = v; kl = (double)K; }
+        if (fabs(v) > fabs(best)) { best = v; bk = K; }
     }
-    a.dscal[1] = err;
-    a.dscal[2] = kl;
+    double e;
+    int ek;
+    block_argmax_abs(best, bk, sv, si, e, ek);
+    if (threadIdx.x == 0) {
+        a.dscal[1] = e;
+        a.dscal[2] = fabs(e) > 0.0 ? (double)ek : (double)a.klocat_in;
+    }
 }
 
 }  // namespace
@@ -158,7 +166,7 @@ int launch_ghosts_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *p
     hipLaunchKernelGGL(k_axis_zaxphi, dim3(g->K), dim3(256), 0, s, a);
     const long total = (long)(g->J + 1) * g->L;
     hipLaunchKernelGGL(k_midplane, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_phichk, dim3(1), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_phichk, dim3(1), dim3(256), 0, s, a);
     return (int)hipGetLastError();
 }
 

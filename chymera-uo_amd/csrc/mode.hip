@@ -20,19 +20,18 @@
 //             64 lanes with cross-lane shuffles
 //   k_zinv    X_q[k][j] = sum_p W_i[k][p] z[q][p][j], written back into the
 //             spectrum (GEMM per mode)
+#include <algorithm>
+
 #include "chypot_impl.h"
 
 namespace chy {
 namespace {
 
-constexpr int TILE = 64;   // output tile edge
-constexpr int KSTEP = 16;  // reduction step
-constexpr int GEMM_THREADS = 256;
-
 struct ModeArgs {
     double *spec;          // [L][K][J]
     double *yh;            // [L][PP][JP]
-    const double *wf, *wi; // [NZ][PP]
+    const double *wf;      // [NZ][PP]  wf[k][p]
+    const double *wi;      // [NZ][PP]  wi[p][k]
     const double *lam;     // [PP]
     const double *am, *cm; // [M]
     const double *bm;      // [N+1][M]
@@ -40,108 +39,236 @@ struct ModeArgs {
     double pig4, cnmax, cmmax;
 };
 
-// yhat[q][p][j] = sum_{k<NZ} wf[k][p] * Y_q[k][j], j < M, p < NZ (other p rows
-// of the tile are written as 0).  Y_q is formed while loading (pot3.f:212-224):
-//   Y = PIG4*S[k][j];  k=NZ-1: Y -= CNMAX*S[NZ][j];  j=M-1: Y -= CMMAX*S[k][M]
-__global__ __launch_bounds__(GEMM_THREADS) void k_zfwd(ModeArgs a)
+typedef double dx4 __attribute__((ext_vector_type(4)));
+
+// z-transform GEMMs on the FP64 matrix cores, one 128x64 output tile per block:
+//   FWD: yhat_q[p][j] = sum_k wf[k][p] Y_q[k][j]   rows p, cols j, reduction k
+//   INV: X_q[k][j]    = sum_p wi[p][k] z_q[p][j]   rows k, cols j, reduction p
+// Both weight tables are stored reduction-index major ([red][row]), so the A
+// tile loads like the B tile: rows of contiguous doubles.  The four
+// wavefronts own 32 rows each (2x4 tiles of 16x16); the next KT-deep slice of
+// A and B is fetched into registers while the current one is multiplied.
+// V_MFMA_F64_16X16X4F64 operands: A lane l holds A[l%16][l/16], B lane l holds
+// B[l/16][l%16]; the result D[row][l%16] with row = MROW(l, v), v = 0..3.
+constexpr int RT = 128, CT = 64, KT = 32;
+constexpr int GEMM_THREADS = 256;
+constexpr int A_PER = KT * RT / GEMM_THREADS;  // 16
+constexpr int B_PER = KT * CT / GEMM_THREADS;  // 8
+
+__device__ __forceinline__ int mrow(int lane, int v) { return (lane >> 4) + 4 * v; }
+
+template <bool FWD>
+__global__ __launch_bounds__(GEMM_THREADS, 2) void k_zgemm(ModeArgs a)
 {
-    __shared__ double As[KSTEP][TILE];      // [k][p]
-    __shared__ double Bs[KSTEP][TILE + 1];  // [k][j]
+    __shared__ double As[KT][RT + 1];
+    __shared__ double Bs[KT][CT];
     const int q = blockIdx.z;
-    const int p0 = blockIdx.y * TILE, j0 = blockIdx.x * TILE;
+    const int r0 = blockIdx.y * RT, c0 = blockIdx.x * CT;
     const int tid = threadIdx.x;
-    const int tp = tid / 16, tj = tid % 16;
-    const size_t JK = (size_t)a.J * a.K;
-    const double *S = a.spec + (size_t)q * JK;
-    double acc[4][4] = {};
-    for (int k0 = 0; k0 < a.NZ; k0 += KSTEP) {
-        for (int e = tid; e < KSTEP * TILE; e += GEMM_THREADS) {
-            const int kk = e / TILE, c = e % TILE;
-            const int k = k0 + kk;
-            const int p = p0 + c, j = j0 + c;
-            As[kk][c] = (k < a.NZ && p < a.PP) ? a.wf[(size_t)k * a.PP + p] : 0.0;
+    const int nred = a.NZ;
+    const int nrow = FWD ? a.PP : a.NZ;
+    const double *W = FWD ? a.wf : a.wi;
+    const double *S = a.spec + (size_t)q * a.J * a.K;
+    const double *Z = a.yh + (size_t)q * a.PP * a.JP;
+
+    double ra[A_PER], rb[B_PER];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < A_PER; ++i) {
+            const int e = tid + GEMM_THREADS * i;
+            const int kk = e / RT, rr = e % RT;
+            const int k = k0 + kk, r = r0 + rr;
+            ra[i] = (k < nred && r < nrow) ? W[(size_t)k * a.PP + r] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < B_PER; ++i) {
+            const int e = tid + GEMM_THREADS * i;
+            const int kk = e / CT, cc = e % CT;
+            const int k = k0 + kk, j = c0 + cc;
             double y = 0.0;
-            if (k < a.NZ && j < a.M) {
-                y = a.pig4 * S[(size_t)k * a.J + j];
-                if (k == a.NZ - 1) y -= a.cnmax * S[(size_t)a.NZ * a.J + j];
-                if (j == a.M - 1) y -= a.cmmax * S[(size_t)k * a.J + a.M];
+            if (k < nred && j < a.M) {
+                if (FWD) {  // Y of pot3.f:212-224
+                    y = a.pig4 * S[(size_t)k * a.J + j];
+                    if (k == a.NZ - 1) y -= a.cnmax * S[(size_t)a.NZ * a.J + j];
+                    if (j == a.M - 1) y -= a.cmmax * S[(size_t)k * a.J + a.M];
+                } else {
+                    y = Z[(size_t)k * a.JP + j];
+                }
             }
-            Bs[kk][c] = y;
+            rb[i] = y;
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int i = 0; i < A_PER; ++i) {
+            const int e = tid + GEMM_THREADS * i;
+            As[e / RT][e % RT] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < B_PER; ++i) {
+            const int e = tid + GEMM_THREADS * i;
+            Bs[e / CT][e % CT] = rb[i];
+        }
+    };
+
+    const int lane = tid & 63, wave = tid >> 6;
+    const int rw = wave * 32, lr = lane & 15, lk = lane >> 4;
+    dx4 acc[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[t][u] = dx4{0.0, 0.0, 0.0, 0.0};
+
+    fetch(0);
+    stash();
+    __syncthreads();
+    for (int k0 = 0; k0 < nred; k0 += KT) {
+        const bool more = k0 + KT < nred;
+        if (more) fetch(k0 + KT);
+#pragma unroll
+        for (int kk = 0; kk < KT; kk += 4) {
+            const double a0 = As[kk + lk][rw + lr], a1 = As[kk + lk][rw + 16 + lr];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double b = Bs[kk + lk][16 * u + lr];
+                acc[0][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b, acc[0][u], 0, 0, 0);
+                acc[1][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b, acc[1][u], 0, 0, 0);
+            }
         }
         __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < KSTEP; ++kk) {
-            double av[4], bv[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = As[kk][tp * 4 + i];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) bv[c] = Bs[kk][tj + 16 * c];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) acc[i][c] = fma(av[i], bv[c], acc[i][c]);
-        }
-        __syncthreads();
-    }
-    double *Y = a.yh + (size_t)q * a.PP * a.JP;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int p = p0 + tp * 4 + i;
-        if (p >= a.PP) continue;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int j = j0 + tj + 16 * c;
-            if (j < a.JP) Y[(size_t)p * a.JP + j] = (p < a.NZ && j < a.M) ? acc[i][c] : 0.0;
+        if (more) {
+            stash();
+            __syncthreads();
         }
     }
+
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = r0 + rw + 16 * t + mrow(lane, v);
+                const int j = c0 + 16 * u + lr;
+                if (FWD) {
+                    if (r < a.PP && j < a.JP)
+                        a.yh[((size_t)q * a.PP + r) * a.JP + j] = (r < a.NZ && j < a.M) ? acc[t][u][v] : 0.0;
+                } else if (r < a.NZ && j < a.M) {
+                    a.spec[((size_t)q * a.K + r) * a.J + j] = acc[t][u][v];
+                }
+            }
 }
 
-// spec[q][k][j] = sum_{p<NZ} wi[k][p] * z[q][p][j] for k < NZ, j < M.
-__global__ __launch_bounds__(GEMM_THREADS) void k_zinv(ModeArgs a)
+// The same products for PP <= 128 (KMAX <= 128) with the weight table held in
+// registers.  One persistent block per CU walks output tiles of PPT rows x 64
+// columns (tile = (q, column block)); wave w owns rows 16w..16w+15, keeps its
+// 16 x PPT slice of the weights as MFMA A operands for the whole run, and
+// takes B operands from a double-buffered LDS tile that the block fills with
+// the next tile's right-hand sides while the current one is multiplied.
+constexpr int PCT = 64;  // columns per tile
+
+template <bool FWD, int PPT>
+__global__ __launch_bounds__(PPT * 4) void k_zgemm_reg(ModeArgs a, int col_tiles, int ntiles)
 {
-    __shared__ double As[KSTEP][TILE + 1];  // [p][k]
-    __shared__ double Bs[KSTEP][TILE];      // [p][j]
-    const int q = blockIdx.z;
-    const int k0t = blockIdx.y * TILE, j0 = blockIdx.x * TILE;
+    constexpr int NT = PPT * 4;            // threads (PPT/16 waves)
+    constexpr int KS = PPT / 4;            // k-steps of the 16x16x4 product
+    constexpr int BPER = PPT * PCT / NT;   // B elements fetched per thread (16)
+    __shared__ double Bs[2][PPT][PCT];
     const int tid = threadIdx.x;
-    const int tk = tid / 16, tj = tid % 16;
-    const double *Z = a.yh + (size_t)q * a.PP * a.JP;
-    double acc[4][4] = {};
-    for (int pp0 = 0; pp0 < a.NZ; pp0 += KSTEP) {
-        for (int e = tid; e < KSTEP * TILE; e += GEMM_THREADS) {
-            // A tile: 64 k rows x 16 p, read along p
-            const int r = e / KSTEP, c = e % KSTEP;
-            const int k = k0t + r, p = pp0 + c;
-            As[c][r] = (k < a.NZ && p < a.NZ) ? a.wi[(size_t)k * a.PP + p] : 0.0;
-            // B tile: 16 p rows x 64 j, read along j
-            const int pb = pp0 + e / TILE, jb = j0 + e % TILE;
-            Bs[e / TILE][e % TILE] = (pb < a.NZ && jb < a.M) ? Z[(size_t)pb * a.JP + jb] : 0.0;
-        }
-        __syncthreads();
+    const int lane = tid & 63, w = tid >> 6, lr = lane & 15, lk = lane >> 4;
+    const int nred = a.NZ;
+    const int nrow = FWD ? a.PP : a.NZ;
+    const double *W = FWD ? a.wf : a.wi;
+
+    int tile = blockIdx.x;
+    if (tile >= ntiles) return;  // the whole block leaves together
+
+    // A operands: A[row][red] = W[red][row], row = 16w + lr, red = 4ks + lk
+    double ar[KS];
 #pragma unroll
-        for (int kk = 0; kk < KSTEP; ++kk) {
-            double av[4], bv[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = As[kk][tk * 4 + i];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) bv[c] = Bs[kk][tj + 16 * c];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) acc[i][c] = fma(av[i], bv[c], acc[i][c]);
-        }
-        __syncthreads();
+    for (int ks = 0; ks < KS; ++ks) {
+        const int kr = 4 * ks + lk, r = 16 * w + lr;
+        ar[ks] = (kr < nred && r < nrow) ? W[(size_t)kr * a.PP + r] : 0.0;
     }
-    double *S = a.spec + (size_t)q * a.J * a.K;
+
+    // fetch() only issues loads (clamped to valid addresses); the right-hand
+    // side is assembled in stash(), after the tile's products, so no load
+    // result is waited for inside the fetch.
+    double rb[BPER], ro[FWD ? BPER : 1], rtop = 0.0;
+    const int cc = tid % PCT;  // this thread's column within every tile
+    auto fetch = [&](int t) {
+        const int q = t / col_tiles, j = (t % col_tiles) * PCT + cc;
+        const int jc = j < a.M ? j : a.M - 1;
+        if (FWD) {
+            const double *S = a.spec + (size_t)q * a.J * a.K;
+            rtop = S[(size_t)a.NZ * a.J + jc];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int k = k0t + tk * 4 + i;
-        if (k >= a.NZ) continue;
+            for (int i = 0; i < BPER; ++i) {
+                const int kr = tid / PCT + (NT / PCT) * i;
+                const int kc = kr < nred ? kr : nred - 1;
+                rb[i] = S[(size_t)kc * a.J + jc];
+                ro[i] = S[(size_t)kc * a.J + a.M];
+            }
+        } else {
+            const double *Z = a.yh + (size_t)q * a.PP * a.JP;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int j = j0 + tj + 16 * c;
-            if (j < a.M) S[(size_t)k * a.J + j] = acc[i][c];
+            for (int i = 0; i < BPER; ++i) {
+                const int kr = tid / PCT + (NT / PCT) * i;
+                const int kc = kr < nred ? kr : nred - 1;
+                rb[i] = Z[(size_t)kc * a.JP + jc];
+            }
         }
+    };
+    auto stash = [&](int buf, int t) {
+        const int j = (t % col_tiles) * PCT + cc;
+#pragma unroll
+        for (int i = 0; i < BPER; ++i) {
+            const int kr = tid / PCT + (NT / PCT) * i;
+            double y = rb[i];
+            if (FWD) {  // Y of pot3.f:212-224
+                y = a.pig4 * y;
+                y = kr == a.NZ - 1 ? y - a.cnmax * rtop : y;
+                y = j == a.M - 1 ? y - a.cmmax * ro[i] : y;
+            }
+            Bs[buf][kr][cc] = (kr < nred && j < a.M) ? y : 0.0;
+        }
+    };
+
+    fetch(tile);
+    stash(0, tile);
+    __syncthreads();
+    int buf = 0;
+    for (; tile < ntiles; tile += gridDim.x) {
+        const int next = tile + gridDim.x;
+        if (next < ntiles) fetch(next);
+        dx4 acc[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = dx4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double b = Bs[buf][4 * ks + lk][16 * u + lr];
+                acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ks], b, acc[u], 0, 0, 0);
+            }
+        }
+        const int q = tile / col_tiles, c0 = (tile % col_tiles) * PCT;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * w + mrow(lane, v);
+                const int j = c0 + 16 * u + lr;
+                if (FWD) {
+                    if (r < a.PP && j < a.JP)
+                        a.yh[((size_t)q * a.PP + r) * a.JP + j] = (r < a.NZ && j < a.M) ? acc[u][v] : 0.0;
+                } else if (r < a.NZ && j < a.M) {
+                    a.spec[((size_t)q * a.K + r) * a.J + j] = acc[u][v];
+                }
+            }
+        if (next < ntiles) stash(buf ^ 1, next);
+        __syncthreads();
+        buf ^= 1;
     }
 }
 
@@ -309,12 +436,30 @@ ModeArgs make_args(const chy_grid *g)
 
 }  // namespace
 
+template <bool FWD>
+void launch_zgemm(const chy_grid *g, const ModeArgs &a, hipStream_t s)
+{
+    const int col_tiles = ((FWD ? g->JP : g->M) + PCT - 1) / PCT;
+    const int ntiles = col_tiles * g->L;
+    const int blocks = std::min(ntiles, g->ncu);
+    if (g->PP == 128) {
+        hipLaunchKernelGGL((k_zgemm_reg<FWD, 128>), dim3(blocks), dim3(512), 0, s, a, col_tiles, ntiles);
+    } else if (g->PP == 64) {
+        hipLaunchKernelGGL((k_zgemm_reg<FWD, 64>), dim3(blocks), dim3(256), 0, s, a, col_tiles, ntiles);
+    } else if (FWD) {
+        dim3 gf((g->JP + CT - 1) / CT, (g->PP + RT - 1) / RT, g->L);
+        hipLaunchKernelGGL(k_zgemm<true>, gf, dim3(GEMM_THREADS), 0, s, a);
+    } else {
+        dim3 gi((g->M + CT - 1) / CT, (g->NZ + RT - 1) / RT, g->L);
+        hipLaunchKernelGGL(k_zgemm<false>, gi, dim3(GEMM_THREADS), 0, s, a);
+    }
+}
+
 int launch_mode_solve(const chy_grid *g, hipStream_t s, Profiler *prof)
 {
     ModeArgs a = make_args(g);
-    dim3 gf((g->JP + TILE - 1) / TILE, (g->PP + TILE - 1) / TILE, g->L);
     if (prof) prof->begin(CHY_STAGE_ZFWD, s);
-    hipLaunchKernelGGL(k_zfwd, gf, dim3(GEMM_THREADS), 0, s, a);
+    launch_zgemm<true>(g, a, s);
     if (prof) prof->end(s);
     const int nsys = (g->N + 1) * g->NZ;
     const int blocks = (nsys + 3) / 4;
@@ -328,9 +473,8 @@ int launch_mode_solve(const chy_grid *g, hipStream_t s, Profiler *prof)
     default: return (int)hipErrorInvalidValue;
     }
     if (prof) prof->end(s);
-    dim3 gi((g->M + TILE - 1) / TILE, (g->NZ + TILE - 1) / TILE, g->L);
     if (prof) prof->begin(CHY_STAGE_ZINV, s);
-    hipLaunchKernelGGL(k_zinv, gi, dim3(GEMM_THREADS), 0, s, a);
+    launch_zgemm<false>(g, a, s);
     if (prof) prof->end(s);
     return (int)hipGetLastError();
 }

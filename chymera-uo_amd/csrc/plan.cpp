@@ -237,6 +237,15 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
     }
     g->isym = isym;
     g->dtheta = desc->dtheta; g->pi = desc->pi; g->grav = desc->grav;
+    {
+        int ncu = 0;
+        hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g->ctx->device);
+        if (e != hipSuccess) {
+            set_device_error(e, "hipDeviceGetAttribute");
+            return CHY_EDEVICE;
+        }
+        g->ncu = ncu > 0 ? ncu : 1;
+    }
     g->r.assign(desc->r, desc->r + J + 2);
     g->z.assign(desc->z, desc->z + K + 2);
     g->rhf.assign(desc->rhf, desc->rhf + J + 2);
@@ -312,7 +321,7 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
         for (int p = 0; p < NZ; ++p) {
             long double qv = q[(size_t)k * NZ + p];
             wf[(size_t)k * PP + p] = (double)(dsc[k] * qv);
-            wi[(size_t)k * PP + p] = (double)(qv / dsc[k]);
+            wi[(size_t)p * PP + k] = (double)(qv / dsc[k]);  // stored [p][k]: reduction index major
         }
 
     // ---- FFT twiddles (exact angles; the reference uses recurrences, fft.f:225-232) ----
@@ -324,6 +333,11 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
     for (int e = 0; e < 64; ++e) {
         double ang = 2.0 * 3.14159265358979323846 * (double)e / 64.0;
         tw64[e] = make_double2(std::cos(ang), std::sin(ang));
+    }
+    std::vector<double2> twh(kMaxTrm + 1);
+    for (int m = 0; m <= kMaxTrm; ++m) {
+        double ang = 3.14159265358979323846 * (double)m / (double)L;
+        twh[m] = make_double2(std::cos(ang), std::sin(ang));
     }
 
     // ---- ZAXPHI abscissae source: RHF(1..11) (pot3.f:342-348) ----
@@ -417,6 +431,7 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
     DeviceGrid &d = g->d;
     CHY_TRY(upload(g, &d.twL, twL));
     CHY_TRY(upload(g, &d.tw64, tw64));
+    CHY_TRY(upload(g, &d.twh, twh));
     CHY_TRY(upload(g, &d.wf, wf));
     CHY_TRY(upload(g, &d.wi, wi));
     CHY_TRY(upload(g, &d.lam, lam));

@@ -123,9 +123,16 @@ int chy_bdygen(chy_grid *g, int maxtrm, int isym, double redge, const double *rh
 int chy_pot3(chy_grid *g, int npoint, const double *rho_d, double *phi_d, double *coef_d,
              int coef_mode, chy_diag *diag);
 
-/* BDYGEN followed by POT3: one potential solve, device-resident, one stream. */
+/* BDYGEN followed by POT3: one potential solve, device-resident, one stream.
+ * Writes the same cells of phi_d as chy_bdygen then chy_pot3 would.  Unless
+ * stage timing is on or graphs are off (chy_ctx_graphs), the kernel sequence
+ * is captured into a HIP graph on the first call with a given set of
+ * arguments (pointers, scalars, stream) and replayed by later calls with the
+ * same set; a call with different arguments re-captures. */
 int chy_solve(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho_d,
               double *phi_d, double *coef_d, int coef_mode, chy_diag *diag);
+/* Enable (default) or disable graph replay of chy_solve on this context. */
+int chy_ctx_graphs(chy_ctx *ctx, int enable);
 
 /* Host-pointer forms used by the Fortran shims: rho_h/phi_h/coef_h are the
  * COMMON arrays.  They copy to the grid's device buffers, run, copy back and
@@ -141,13 +148,14 @@ int chy_pot3_host(chy_grid *g, int npoint, const double *rho_h, double *phi_h, d
  * stream and returns, per stage, the summed milliseconds and the number of
  * launches since profiling was (re)enabled. */
 enum {
-    CHY_STAGE_BDYGEN = 0, /* ring moments, radial split, boundary evaluation */
-    CHY_STAGE_FFT_FWD,    /* forward azimuthal RFFT (+ COEF) */
+    CHY_STAGE_BDYGEN = 0, /* radial split sums, boundary evaluation, ghosts */
+    CHY_STAGE_FFT_FWD,    /* forward azimuthal RFFT (+ COEF, + BDYGEN ring moments) */
     CHY_STAGE_ZFWD,       /* mode right-hand sides -> z eigen coordinates */
     CHY_STAGE_RSOLVE,     /* r-tridiagonal solves */
     CHY_STAGE_ZINV,       /* z eigen coordinates -> mode solutions */
     CHY_STAGE_FFT_INV,    /* inverse azimuthal RFFT */
     CHY_STAGE_GHOST,      /* ghosts, ZAXPHI, PHICHK */
+    CHY_STAGE_FFT_BDY,    /* forward RFFT of the boundary lanes (chy_solve) */
     CHY_NSTAGES
 };
 int chy_ctx_profile(chy_ctx *ctx, int enable);

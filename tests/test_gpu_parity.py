@@ -136,3 +136,107 @@ def test_bad_arguments(ctx):
     with pytest.raises(chypot.ChyError) as ei:
         pg.bdygen(rho, phi, isym=1)
     assert ei.value.code == chypot.CHY_EINVAL
+
+
+def fused_solve(ctx, g, rho_d, params, coef_mode=chypot.COEF_FULL, phi_init=0.0, pg=None):
+    own = pg is None
+    if own:
+        pg = chypot.PreparedGrid(ctx, g, isym=params["isym"])
+    phi_d = torch.full_like(rho_d, phi_init)
+    coef = None
+    if coef_mode != chypot.COEF_NONE:
+        coef = torch.zeros((2, g.lmax // 2, g.kmax + 2, g.jmax + 2), dtype=torch.float64, device="cuda")
+    d = pg.solve(rho_d, phi_d, maxtrm=params["maxtrm"], isym=params["isym"], redge=params["redge"],
+                 npoint=params["npoint"], coef=coef, coef_mode=coef_mode, diag=True)
+    ctx.sync()
+    out = dict(phi=chypot.to_host(phi_d), tmass=d.tmass, phichk=d.phichk, klocat=d.klocat)
+    if coef is not None:
+        out["coef"] = np.asfortranarray(coef.cpu().numpy().transpose(3, 2, 1, 0))
+    if own:
+        pg.close()
+    return out
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+@pytest.mark.parametrize("name", SMALL_CASES)
+def test_fused_solve_matches_reference(ctx, name, graphs):
+    """chy_solve (BDYGEN evaluated in the azimuthal spectrum when LMAX/2 > 10,
+    through PHI otherwise) against the reference's BDYGEN+POT3."""
+    g, rho, params, f = load_case(name)
+    print(f"[gpu] fused solve {name} {g.name} graphs={graphs}", flush=True)
+    ctx.graphs(graphs)
+    try:
+        res = fused_solve(ctx, g, chypot.to_device(rho), params, phi_init=3.5)
+    finally:
+        ctx.graphs(True)
+    # the cells POT3 does not write keep their contents (the fixture holds 0 there)
+    phi = res["phi"]
+    assert np.all(phi[:, g.kmax + 1, :] == 3.5)
+    assert np.all(phi[g.jmax + 1, 0, :] == 3.5)
+    phi[:, g.kmax + 1, :] = 0.0
+    phi[g.jmax + 1, 0, :] = 0.0
+    assert rel_max(phi, f["phi"]) < 1e-12
+    assert abs(res["tmass"] - float(f["tmass_b"])) <= 1e-13 * abs(float(f["tmass_b"]))
+    assert abs(res["phichk"] - float(f["phichk"])) < 1e-10
+    assert res["klocat"] == int(f["klocat"])
+    c, rk = res["coef"], f["coef_k2"]
+    assert np.abs(c[:, 1, :, 0] - rk[..., 0]).max() < 1e-11
+    sig = rk[..., 0] > 1e-8
+    assert np.abs(phase_diff(c[:, 1, :, 1], rk[..., 1])[sig]).max() < 1e-6
+    if "coef" in f.files:
+        rc = f["coef"]
+        assert np.abs(c[..., 0] - rc[..., 0]).max() < 1e-11
+        sig = rc[..., 0] > 1e-8
+        assert np.abs(phase_diff(c[..., 1], rc[..., 1])[sig]).max() < 1e-6
+
+
+def test_solve_graph_replays_match_eager(ctx):
+    """Replays of the captured solve graph, with eager solves, other GPU work
+    and host waits between them, each equal bit for bit to the eager solve of
+    the density the buffer holds at replay time; a call with another output
+    buffer re-captures."""
+    g, rho, params, f = load_case("c1_64x32x64")
+    rho2 = fields.density_spiral_torus(g, seed=5)
+    kw = dict(maxtrm=params["maxtrm"], isym=params["isym"], redge=params["redge"], npoint=params["npoint"])
+    pg = chypot.PreparedGrid(ctx, g, isym=params["isym"])
+    eager = {}
+    ctx.graphs(False)
+    for key, r in (("a", rho), ("b", rho2)):
+        rd = chypot.to_device(r)
+        ph = torch.zeros_like(rd)
+        cf = torch.zeros((2, g.lmax // 2, g.kmax + 2, g.jmax + 2), dtype=torch.float64, device="cuda")
+        pg.solve(rd, ph, coef=cf, coef_mode=chypot.COEF_FULL, **kw)
+        ctx.sync()
+        eager[key] = (ph.cpu(), cf.cpu())
+    ctx.graphs(True)
+    rho_d = chypot.to_device(rho)
+    phi_d = torch.zeros_like(rho_d)
+    coef = torch.zeros((2, g.lmax // 2, g.kmax + 2, g.jmax + 2), dtype=torch.float64, device="cuda")
+    try:
+        for it, key in enumerate(("a", "a", "b", "a", "b", "b")):
+            print(f"[gpu] graph replay {it} density {key}", flush=True)
+            rho_d.copy_(chypot.to_device(rho if key == "a" else rho2))
+            phi_d.fill_(0.0)
+            coef.fill_(0.0)
+            pg.solve(rho_d, phi_d, coef=coef, coef_mode=chypot.COEF_FULL, **kw)
+            ctx.sync()
+            assert float(phi_d.abs().max()) > 0.0, it
+            assert torch.equal(phi_d.cpu(), eager[key][0]), it
+            assert torch.equal(coef.cpu(), eager[key][1]), it
+            # eager work between replays: an un-captured solve on the same grid
+            # and a torch reduction
+            ctx.graphs(False)
+            tmp = torch.zeros_like(phi_d)
+            pg.solve(rho_d, tmp, **kw)
+            ctx.graphs(True)
+            ctx.sync()
+            assert float(tmp.abs().sum()) > 0.0
+        # another output buffer: a fresh capture, same numbers
+        phi2 = torch.zeros_like(rho_d)
+        pg.solve(rho_d, phi2, coef=coef, coef_mode=chypot.COEF_FULL, **kw)
+        ctx.sync()
+        assert torch.equal(phi2.cpu(), eager["b"][0])
+    finally:
+        ctx.graphs(True)
+        pg.close()
+    assert rel_max(chypot.to_host(eager["a"][0]), f["phi"]) < 1e-12

@@ -1,0 +1,86 @@
+"""CPU-side checks of the product: the C-ABI library loads and exports every
+declared entry point, host logic (grids, synthetic fields) matches what the
+fixtures were made from, and the entry points fail cleanly without a device."""
+import hashlib
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+import chypot
+from chypot import fields
+from helpers import load_case, meta
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "chypot.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char \*)\s*\*?\s*(chy_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = chypot.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 16
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    assert sorted(chypot.EXPORTS) == syms
+
+
+def test_strerror_covers_codes():
+    for code in range(7):
+        assert chypot.lib().chy_strerror(code)
+    assert b"unknown" in chypot.lib().chy_strerror(99)
+
+
+def test_context_without_device_fails_cleanly():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(chypot.ChyError) as ei:
+        chypot.Context(0)
+    assert ei.value.code == chypot.CHY_EINVAL
+
+
+def test_grid_arrays_follow_io_f():
+    g = fields.unit_grid(16, 8, 16)
+    assert np.allclose(g.r()[:3], [-1 / 16, 0.0, 1 / 16])
+    assert np.allclose(g.rhf()[1], 0.5 / 16)
+    assert np.allclose(g.zhf()[1], 0.25 / 8)
+    assert abs(g.dtheta - 2 * np.pi / 16) < 1e-15
+
+
+@pytest.mark.parametrize("name", ["c1_64x32x64", "large_128x64x128", "large_256x128x256"])
+def test_regenerated_densities_match_fixtures(name):
+    md = meta()[name]
+    J, K, L = (int(v) for v in md["grid"].split("x"))
+    g = fields.Grid(J, K, L, md["delr"], md["delz"])
+    rho = fields.density_small_torus(g, 1) if md["density"] == "small" else fields.density_spiral_torus(g, 2)
+    digest = hashlib.sha256(np.asarray(rho, dtype="<f8").tobytes(order="F")).hexdigest()
+    assert digest == md["rho_sha256"]
+
+
+def test_density_floor_and_support():
+    g, rho, p, f = load_case("g16_small")
+    inner = rho[1:g.jmax + 1, 1:g.kmax + 1, :]
+    assert inner.min() >= 1e-10 * inner.max() * (1 - 1e-15)
+    assert np.all(rho[0] == 0) and np.all(rho[:, 0] == 0)
+    assert np.all(rho[g.jmax + 1] == 0) and np.all(rho[:, g.kmax + 1] == 0)
+
+
+def test_random_field_draws_documented_ranges():
+    seen = set()
+    for s in range(12):
+        g, rho = fields.random_field(s)
+        assert g.jmax in (64, 128, 256) and g.kmax in (32, 64, 128) and g.lmax in (64, 128, 256)
+        assert rho.shape == g.shape and rho.flags.f_contiguous
+        seen.add(g.name)
+    assert len(seen) > 3
+
+
+def test_host_arrays_must_be_fortran_float64():
+    with pytest.raises(ValueError):
+        chypot._host_ok(np.zeros((3, 3, 3)))
+    chypot._host_ok(np.zeros((3, 3, 3), order="F"))
