@@ -33,6 +33,7 @@ EXPORTS = (
     "chy_setbdy", "chy_grid_destroy", "chy_grid_device_bytes",
     "chy_bdygen", "chy_pot3", "chy_solve", "chy_bdygen_host", "chy_pot3_host",
     "chy_ctx_profile", "chy_ctx_stage_times", "chy_ctx_graphs",
+    "chy_zaxphi", "chy_zaxphi_host", "chy_phac", "chy_solve_batch", "chy_ctx_workers",
     "chy_strerror", "chy_last_device_error",
 )
 
@@ -55,6 +56,13 @@ class GridDesc(C.Structure):
 
 class Diag(C.Structure):
     _fields_ = [("tmass", C.c_double), ("phichk", C.c_double), ("klocat", C.c_int)]
+
+
+class SolveItem(C.Structure):
+    _fields_ = [("grid", C.c_void_p), ("rho_d", C.c_void_p), ("phi_d", C.c_void_p),
+                ("coef_d", C.c_void_p), ("coef_mode", C.c_int), ("maxtrm", C.c_int),
+                ("isym", C.c_int), ("npoint", C.c_int), ("redge", C.c_double),
+                ("diag", C.POINTER(Diag))]
 
 
 _lib = None
@@ -84,6 +92,11 @@ def lib():
         L.chy_pot3_host.argtypes = [vp, C.c_int, dp, dp, dp, C.c_int, C.POINTER(Diag)]
         L.chy_ctx_profile.argtypes = [vp, C.c_int]
         L.chy_ctx_graphs.argtypes = [vp, C.c_int]
+        L.chy_zaxphi.argtypes = [vp, C.c_int, vp, C.POINTER(Diag)]
+        L.chy_zaxphi_host.argtypes = [vp, C.c_int, dp, C.POINTER(Diag)]
+        L.chy_phac.argtypes = [vp, dp, C.c_int]
+        L.chy_solve_batch.argtypes = [vp, C.POINTER(SolveItem), C.c_int]
+        L.chy_ctx_workers.argtypes = [vp, C.c_int]
         L.chy_ctx_stage_times.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int]
         L.chy_strerror.argtypes = [C.c_int]
         L.chy_strerror.restype = C.c_char_p
@@ -131,6 +144,28 @@ class Context:
     def graphs(self, enable: bool = True):
         """Replay chy_solve from captured HIP graphs (default on)."""
         _check(lib().chy_ctx_graphs(self.handle, int(enable)), "chy_ctx_graphs")
+
+    def workers(self, n: int):
+        """Worker streams used by solve_batch (1..16)."""
+        _check(lib().chy_ctx_workers(self.handle, int(n)), "chy_ctx_workers")
+
+    def solve_batch(self, jobs, diag=False):
+        """Solve independent fields in one call.  jobs: iterable of dicts with
+        keys grid (PreparedGrid), rho, phi and optionally coef, coef_mode,
+        maxtrm, isym, redge, npoint.  Returns a list of Diag (diag=True, waits)
+        or None (asynchronous on this context's stream)."""
+        jobs = list(jobs)
+        items = (SolveItem * len(jobs))()
+        diags = [Diag() for _ in jobs] if diag else None
+        for i, jb in enumerate(jobs):
+            pg = jb["grid"]
+            items[i] = SolveItem(pg.handle.value, _dptr(jb["rho"]).value, _dptr(jb["phi"]).value,
+                                 _dptr(jb.get("coef")).value if jb.get("coef") is not None else None,
+                                 jb.get("coef_mode", COEF_NONE), jb.get("maxtrm", 10),
+                                 jb.get("isym", pg.isym), jb.get("npoint", 8), jb.get("redge", 0.0),
+                                 C.pointer(diags[i]) if diag else None)
+        _check(lib().chy_solve_batch(self.handle, items, len(jobs)), "chy_solve_batch")
+        return diags
 
     def stage_times(self):
         """{stage: (total_ms, launches)} since profiling was enabled (waits)."""
@@ -193,6 +228,19 @@ class PreparedGrid:
                                _dptr(rho), _dptr(phi), _dptr(coef), coef_mode,
                                C.byref(d) if d else None), "chy_solve")
         return d
+
+    def zaxphi(self, phi, npoint=8, klocat=0):
+        """ZAXPHI alone on a device PHI (pot3.f:311-399); returns the diagnostics."""
+        d = Diag(0.0, 0.0, klocat)
+        _check(lib().chy_zaxphi(self.handle, npoint, _dptr(phi), C.byref(d)), "chy_zaxphi")
+        return d
+
+    def phac(self):
+        """PHAC(K), K=2..KMAX+1, of the last POT3/ZAXPHI on this grid."""
+        out = np.zeros(self.grid.kmax)
+        _check(lib().chy_phac(self.handle, out.ctypes.data_as(C.POINTER(C.c_double)), self.grid.kmax),
+               "chy_phac")
+        return out
 
     # -- host forms (numpy Fortran-ordered float64 arrays, updated in place) --
     def bdygen_host(self, rho, phi, maxtrm=10, isym=None, redge=0.0):

@@ -2,11 +2,13 @@
 // points and their host-pointer forms.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "chypot_impl.h"
 
@@ -119,6 +121,18 @@ static bool bdygen_args_ok(int maxtrm, int isym, double redge)
     return maxtrm >= 0 && maxtrm <= kMaxTrm && std::abs(isym) == 2 && redge >= 0.0;
 }
 
+// waits for the entry's last launch, then frees it
+static void retire_graph(chy_grid::GraphEntry &e)
+{
+    if (e.done) {
+        (void)hipEventSynchronize(e.done);
+        (void)hipEventDestroy(e.done);
+    }
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    e.done = nullptr;
+    e.exec = nullptr;
+}
+
 // BDYGEN alone: ring moments from a transform of the interior lanes (no
 // spectrum kept), then the radial sums and the boundary values.
 static int run_bdygen(chy_grid *g, int maxtrm, int isym, double redge, const double *rho, double *phi)
@@ -139,9 +153,8 @@ static int run_bdygen(chy_grid *g, int maxtrm, int isym, double redge, const dou
 }
 
 // the solve after the forward transform: mode solves, inverse transform, ghosts, ZAXPHI
-static int run_pot3_tail(chy_grid *g, int npoint, double *phi, int klocat_in)
+static int run_pot3_tail(chy_grid *g, hipStream_t s, int npoint, double *phi, int klocat_in)
 {
-    hipStream_t s = g->ctx->stream;
     Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
     int rc = dev_rc(launch_mode_solve(g, s, p), "mode solve launch");
     if (rc) return rc;
@@ -150,7 +163,7 @@ static int run_pot3_tail(chy_grid *g, int npoint, double *phi, int klocat_in)
     if (p) p->end(s);
     if (rc) return rc;
     if (p) p->begin(CHY_STAGE_GHOST, s);
-    rc = dev_rc(launch_ghosts_zaxphi(g, npoint, klocat_in, phi, s), "ghost/ZAXPHI launch");
+    rc = dev_rc(launch_ghosts_zaxphi(g, npoint, klocat_in, phi, true, s), "ghost/ZAXPHI launch");
     if (p) p->end(s);
     return rc;
 }
@@ -159,10 +172,9 @@ static int run_pot3_tail(chy_grid *g, int npoint, double *phi, int klocat_in)
 // both the spectrum and the ring moments; BDYGEN then writes the boundary
 // lanes' spectrum directly (a Fourier series of order <= MAXTRM).  For
 // LMAX/2 <= 10 the boundary values go through PHI and a transform instead.
-static int run_solve_eager(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho,
-                           double *phi, double *coef, int coef_mode, int klocat_in)
+static int run_solve_eager(chy_grid *g, hipStream_t s, int maxtrm, int isym, double redge, int npoint,
+                           const double *rho, double *phi, double *coef, int coef_mode, int klocat_in)
 {
-    hipStream_t s = g->ctx->stream;
     Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
     const bool spectral = bdygen_spectral_ok(g);
     if (p) p->begin(CHY_STAGE_FFT_FWD, s);
@@ -182,43 +194,56 @@ static int run_solve_eager(chy_grid *g, int maxtrm, int isym, double redge, int 
         if (p) p->end(s);
         if (rc) return rc;
     }
-    return run_pot3_tail(g, npoint, phi, klocat_in);
+    return run_pot3_tail(g, s, npoint, phi, klocat_in);
 }
 
 // The same sequence replayed from a HIP graph (captured on first use for a
 // given set of arguments) when stage timing is off: one launch per solve.
-static int run_solve(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho,
-                     double *phi, double *coef, int coef_mode, int klocat_in)
+static int run_solve(chy_grid *g, hipStream_t s, int maxtrm, int isym, double redge, int npoint,
+                     const double *rho, double *phi, double *coef, int coef_mode, int klocat_in)
 {
     if (!bdygen_args_ok(maxtrm, isym, redge)) return CHY_EINVAL;
     chy_ctx *c = g->ctx;
     if (c->prof.on || !c->graphs)
-        return run_solve_eager(g, maxtrm, isym, redge, npoint, rho, phi, coef, coef_mode, klocat_in);
-    SolveKey key{rho, phi, coef, coef_mode, maxtrm, isym, npoint, klocat_in, redge, c->stream};
-    if (!g->graph_exec || !(g->graph_key == key)) {
-        if (g->graph_exec) {
-            (void)hipGraphExecDestroy(g->graph_exec);
-            g->graph_exec = nullptr;
+        return run_solve_eager(g, s, maxtrm, isym, redge, npoint, rho, phi, coef, coef_mode, klocat_in);
+    SolveKey key{rho, phi, coef, coef_mode, maxtrm, isym, npoint, klocat_in, redge, s};
+    hipGraphExec_t exec = nullptr;
+    for (size_t i = 0; i < g->graphs.size(); ++i) {
+        if (g->graphs[i].key == key) {
+            chy_grid::GraphEntry hit = g->graphs[i];
+            g->graphs.erase(g->graphs.begin() + (long)i);
+            g->graphs.push_back(hit);
+            exec = hit.exec;
+            break;
         }
+    }
+    if (!exec) {
         hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return dev_rc((int)e, "hipStreamBeginCapture");
-        int rc = run_solve_eager(g, maxtrm, isym, redge, npoint, rho, phi, coef, coef_mode, klocat_in);
-        e = hipStreamEndCapture(c->stream, &graph);
+        int rc = run_solve_eager(g, s, maxtrm, isym, redge, npoint, rho, phi, coef, coef_mode, klocat_in);
+        e = hipStreamEndCapture(s, &graph);
         if (rc) {
             if (graph) (void)hipGraphDestroy(graph);
             return rc;
         }
         if (e != hipSuccess) return dev_rc((int)e, "hipStreamEndCapture");
-        e = hipGraphInstantiate(&g->graph_exec, graph, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return dev_rc((int)e, "hipGraphInstantiate");
+        hipEvent_t done = nullptr;
+        e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
         if (e != hipSuccess) {
-            g->graph_exec = nullptr;
-            return dev_rc((int)e, "hipGraphInstantiate");
+            (void)hipGraphExecDestroy(exec);
+            return dev_rc((int)e, "hipEventCreate");
         }
-        g->graph_key = key;
+        if ((int)g->graphs.size() >= kGraphCache) retire_graph(g->graphs.front());
+        if ((int)g->graphs.size() >= kGraphCache) g->graphs.erase(g->graphs.begin());
+        g->graphs.push_back(chy_grid::GraphEntry{key, exec, done});
     }
-    return dev_rc((int)hipGraphLaunch(g->graph_exec, c->stream), "hipGraphLaunch");
+    int rc = dev_rc((int)hipGraphLaunch(exec, s), "hipGraphLaunch");
+    if (rc == CHY_OK) rc = dev_rc((int)hipEventRecord(g->graphs.back().done, s), "hipEventRecord");
+    return rc;
 }
 
 static int run_pot3(chy_grid *g, int npoint, const double *rho, double *phi, double *coef, int coef_mode,
@@ -231,7 +256,7 @@ static int run_pot3(chy_grid *g, int npoint, const double *rho, double *phi, dou
                     "forward FFT launch");
     if (p) p->end(s);
     if (rc) return rc;
-    return run_pot3_tail(g, npoint, phi, klocat_in);
+    return run_pot3_tail(g, s, npoint, phi, klocat_in);
 }
 
 static int read_diag(chy_grid *g, chy_diag *diag, bool tmass, bool zax)
@@ -258,6 +283,33 @@ static int ensure_staging(chy_grid *g, bool coef)
     if (!g->d.h_phi && hipMalloc(&g->d.h_phi, fb) != hipSuccess) return CHY_ENOMEM;
     if (coef && !g->d.h_coef && hipMalloc(&g->d.h_coef, sizeof(double) * coef_elems(g)) != hipSuccess)
         return CHY_ENOMEM;
+    return CHY_OK;
+}
+
+static void release_workers(chy_ctx *c)
+{
+    for (hipStream_t w : c->workers) (void)hipStreamDestroy(w);
+    for (hipEvent_t e : c->worker_ev) (void)hipEventDestroy(e);
+    c->workers.clear();
+    c->worker_ev.clear();
+}
+
+static int ensure_workers(chy_ctx *c)
+{
+    if (!c->fork_ev && hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess)
+        return CHY_EDEVICE;
+    while ((int)c->workers.size() < c->nworkers) {
+        hipStream_t w = nullptr;
+        hipEvent_t e = nullptr;
+        hipError_t err = hipStreamCreateWithFlags(&w, hipStreamNonBlocking);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        if (err != hipSuccess) {
+            if (w) (void)hipStreamDestroy(w);
+            return dev_rc((int)err, "worker stream");
+        }
+        c->workers.push_back(w);
+        c->worker_ev.push_back(e);
+    }
     return CHY_OK;
 }
 
@@ -309,8 +361,100 @@ int chy_ctx_destroy(chy_ctx *ctx)
     if (!ctx) return CHY_EINVAL;
     DeviceScope ds(ctx->device);
     ctx->prof.collect();
+    release_workers(ctx);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->d_diag) (void)hipFree(ctx->d_diag);
+    if (ctx->h_diag) (void)hipHostFree(ctx->h_diag);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
+    return CHY_OK;
+}
+
+int chy_ctx_workers(chy_ctx *ctx, int n)
+{
+    if (!ctx || n < 1 || n > CHY_MAX_WORKERS) return CHY_EINVAL;
+    DeviceScope ds(ctx->device);
+    if (n == ctx->nworkers) return CHY_OK;
+    int rc = dev_rc((int)hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    for (hipStream_t w : ctx->workers)
+        if (rc == CHY_OK) rc = dev_rc((int)hipStreamSynchronize(w), "hipStreamSynchronize");
+    if (rc) return rc;
+    release_workers(ctx);
+    ctx->nworkers = n;
+    return CHY_OK;
+}
+
+int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n)
+{
+    if (!ctx || n < 0 || (n > 0 && !items)) return CHY_EINVAL;
+    if (n == 0) return CHY_OK;
+    // validate everything before anything is launched
+    bool any_diag = false;
+    for (int i = 0; i < n; ++i) {
+        const chy_solve_item &it = items[i];
+        if (!it.grid) return CHY_ESTATE;
+        if (it.grid->ctx != ctx || !it.rho_d || !it.phi_d) return CHY_EINVAL;
+        int np = it.npoint;
+        int rc = check_pot3_args(it.grid, np, it.coef_mode, it.coef_d);
+        if (rc) return rc;
+        if (!bdygen_args_ok(it.maxtrm, it.isym, it.redge)) return CHY_EINVAL;
+        any_diag = any_diag || it.diag;
+    }
+    DeviceScope ds(ctx->device);
+    int rc = ensure_workers(ctx);
+    if (rc) return rc;
+    if (any_diag && ctx->diag_cap < n) {
+        if (ctx->d_diag) (void)hipFree(ctx->d_diag);
+        if (ctx->h_diag) (void)hipHostFree(ctx->h_diag);
+        ctx->d_diag = ctx->h_diag = nullptr;
+        ctx->diag_cap = 0;
+        if (hipMalloc(&ctx->d_diag, sizeof(double) * 3 * n) != hipSuccess) return CHY_ENOMEM;
+        if (hipHostMalloc(&ctx->h_diag, sizeof(double) * 3 * n) != hipSuccess) return CHY_ENOMEM;
+        ctx->diag_cap = n;
+    }
+    // grids are dealt to the worker streams in order of first appearance;
+    // every solve on one grid runs on that grid's stream (they share its
+    // workspace), solves on different grids may overlap
+    std::vector<chy_grid *> seen;
+    std::vector<int> lane(n);
+    for (int i = 0; i < n; ++i) {
+        int k = 0;
+        while (k < (int)seen.size() && seen[k] != items[i].grid) ++k;
+        if (k == (int)seen.size()) seen.push_back(items[i].grid);
+        lane[i] = k % ctx->nworkers;
+    }
+    const int used = std::min((int)seen.size(), ctx->nworkers);
+    rc = dev_rc((int)hipEventRecord(ctx->fork_ev, ctx->stream), "hipEventRecord");
+    for (int w = 0; w < used && rc == CHY_OK; ++w)
+        rc = dev_rc((int)hipStreamWaitEvent(ctx->workers[w], ctx->fork_ev, 0), "hipStreamWaitEvent");
+    for (int i = 0; i < n && rc == CHY_OK; ++i) {
+        const chy_solve_item &it = items[i];
+        hipStream_t s = ctx->workers[lane[i]];
+        int np = std::min(it.npoint, 10);
+        rc = run_solve(it.grid, s, it.maxtrm, it.isym, it.redge, np, it.rho_d, it.phi_d, it.coef_d,
+                       it.coef_mode, it.diag ? it.diag->klocat : 0);
+        if (rc == CHY_OK && it.diag)
+            rc = dev_rc((int)hipMemcpyAsync(ctx->d_diag + 3 * i, it.grid->d.dscal, 3 * sizeof(double),
+                                            hipMemcpyDeviceToDevice, s), "diagnostics copy");
+    }
+    // join: the context's stream waits for every worker it forked
+    for (int w = 0; w < used; ++w) {
+        int r2 = dev_rc((int)hipEventRecord(ctx->worker_ev[w], ctx->workers[w]), "hipEventRecord");
+        if (r2 == CHY_OK) r2 = dev_rc((int)hipStreamWaitEvent(ctx->stream, ctx->worker_ev[w], 0),
+                                      "hipStreamWaitEvent");
+        if (rc == CHY_OK) rc = r2;
+    }
+    if (rc || !any_diag) return rc;
+    hipError_t e = hipMemcpyAsync(ctx->h_diag, ctx->d_diag, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                                  ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return dev_rc((int)e, "diagnostics readback");
+    for (int i = 0; i < n; ++i) {
+        if (!items[i].diag) continue;
+        items[i].diag->tmass = ctx->h_diag[3 * i];
+        items[i].diag->phichk = ctx->h_diag[3 * i + 1];
+        items[i].diag->klocat = (int)ctx->h_diag[3 * i + 2];
+    }
     return CHY_OK;
 }
 
@@ -379,8 +523,8 @@ int chy_grid_destroy(chy_grid *g)
     if (!g) return CHY_EINVAL;
     DeviceScope ds(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
-    if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
-    g->graph_exec = nullptr;
+    for (auto &e : g->graphs) retire_graph(e);
+    g->graphs.clear();
     free_grid_memory(g);
     delete g;
     return CHY_OK;
@@ -420,7 +564,8 @@ int chy_solve(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const
     int rc = check_pot3_args(g, npoint, coef_mode, coef_d);
     if (rc) return rc;
     DeviceScope ds(g->ctx->device);
-    rc = run_solve(g, maxtrm, isym, redge, npoint, rho_d, phi_d, coef_d, coef_mode, diag ? diag->klocat : 0);
+    rc = run_solve(g, g->ctx->stream, maxtrm, isym, redge, npoint, rho_d, phi_d, coef_d, coef_mode,
+                   diag ? diag->klocat : 0);
     if (rc == CHY_OK && diag) rc = read_diag(g, diag, true, true);
     return rc;
 }
@@ -472,6 +617,56 @@ int chy_pot3_host(chy_grid *g, int npoint, const double *rho_h, double *phi_h, d
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return dev_rc((int)e, "device to host copy");
     return diag ? read_diag(g, diag, false, true) : CHY_OK;
+}
+
+int chy_zaxphi(chy_grid *g, int npoint, double *phi_d, chy_diag *diag)
+{
+    if (!g) return CHY_ESTATE;
+    if (!phi_d) return CHY_EINVAL;
+    int rc = check_pot3_args(g, npoint, CHY_COEF_NONE, nullptr);
+    if (rc) return rc;
+    DeviceScope ds(g->ctx->device);
+    hipStream_t s = g->ctx->stream;
+    Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
+    if (p) p->begin(CHY_STAGE_GHOST, s);
+    rc = dev_rc(launch_ghosts_zaxphi(g, npoint, diag ? diag->klocat : 0, phi_d, false, s), "ZAXPHI launch");
+    if (p) p->end(s);
+    if (rc == CHY_OK && diag) rc = read_diag(g, diag, false, true);
+    return rc;
+}
+
+int chy_zaxphi_host(chy_grid *g, int npoint, double *phi_h, chy_diag *diag)
+{
+    if (!g) return CHY_ESTATE;
+    if (!phi_h) return CHY_EINVAL;
+    int rc = check_pot3_args(g, npoint, CHY_COEF_NONE, nullptr);
+    if (rc) return rc;
+    DeviceScope ds(g->ctx->device);
+    rc = ensure_staging(g, false);
+    if (rc) return rc;
+    hipStream_t s = g->ctx->stream;
+    const size_t fb = sizeof(double) * field_elems(g);
+    hipError_t e = hipMemcpyAsync(g->d.h_phi, phi_h, fb, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return dev_rc((int)e, "host to device copy");
+    rc = dev_rc(launch_ghosts_zaxphi(g, npoint, diag ? diag->klocat : 0, g->d.h_phi, false, s), "ZAXPHI launch");
+    if (rc) return rc;
+    e = hipMemcpyAsync(phi_h, g->d.h_phi, fb, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return dev_rc((int)e, "device to host copy");
+    return diag ? read_diag(g, diag, false, true) : CHY_OK;
+}
+
+int chy_phac(chy_grid *g, double *phac_h, int n)
+{
+    if (!g) return CHY_ESTATE;
+    if (!phac_h || n < 0 || n > g->K) return CHY_EINVAL;
+    if (n == 0) return CHY_OK;
+    DeviceScope ds(g->ctx->device);
+    hipStream_t s = g->ctx->stream;
+    // PHAC(K), K = 2..KMAX+1, lives at dscal[4 + K]
+    hipError_t e = hipMemcpyAsync(phac_h, g->d.dscal + 6, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return dev_rc((int)e, "PHAC readback");
 }
 
 }  // extern "C"

@@ -34,8 +34,8 @@ constexpr int CHUNK_GROUPS = 7;      // ring groups per chunk block (7 x 36 <= 2
 struct BdyArgs {
     double *phi;
     const double *cosm, *sinm;
-    const double *mom;
-    const int *ring_of_pos;
+    const double *mom;            // [kMoments][nring] by ring index
+    const int *ring_of_pos;       // radius-sorted position -> ring index
     const double *tin, *tout, *rsph;
     const int *chunk_start;
     double *chunk;
@@ -48,32 +48,44 @@ struct BdyArgs {
     double redge, grav, pi;
 };
 
-// chunk[c][0][2t+s]: sum over the chunk's rings of TERM_t * moment_s (inside
-// form), chunk[c][1][..]: the same with TERM_t r^-(2l+1) (outside form);
+// chunk[0][2t+s][c]: sum over chunk c's rings of TERM_t * moment_s (inside
+// form), chunk[1][2t+s][c]: the same with TERM_t r^-(2l+1) (outside form);
 // s=0 cosine (SUMASS for m=0), s=1 sine.
 __global__ __launch_bounds__(256) void k_bdy_chunks(BdyArgs a)
 {
+    constexpr int IT = (kChunkRings + CHUNK_GROUPS - 1) / CHUNK_GROUPS;  // rings per thread
     __shared__ double part[CHUNK_GROUPS][kMaxTerms][4];
     const int c = blockIdx.x;
     const int tid = threadIdx.x;
     const int p0 = a.chunk_start[c], p1 = a.chunk_start[c + 1];
     if (tid < CHUNK_GROUPS * kMaxTerms) {
         const int t = tid % kMaxTerms, grp = tid / kMaxTerms;
+        const int m = kTermM[t];
+        // every load first (positions clamped into the chunk), then the sums
+        // in ascending position order
+        double mc[IT], ms[IT], ti[IT], to[IT], rs[IT];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int pos = p0 + grp + CHUNK_GROUPS * it;
+            const int pc = pos < p1 ? pos : p0;
+            const double *mo = a.mom + a.ring_of_pos[pc];
+            mc[it] = mo[(size_t)m * a.nring];         // SUMASS (m=0) or CM(m)
+            ms[it] = mo[(size_t)(10 + m) * a.nring];  // SM(m) for m >= 1
+            ti[it] = a.tin[(size_t)pc * kMaxTerms + t];
+            to[it] = a.tout[(size_t)pc * kMaxTerms + t];
+            rs[it] = a.rsph[pc];
+        }
         double ic = 0.0, is = 0.0, oc = 0.0, os = 0.0;
-        if (t < a.nt) {
-            const int m = kTermM[t];
-            for (int pos = p0 + grp; pos < p1; pos += CHUNK_GROUPS) {
-                if (a.redge > 0.0 && a.rsph[pos] > a.redge) continue;  // boundary.f:379
-                const double *mo = a.mom + (size_t)a.ring_of_pos[pos] * kMoments;
-                const double mc = mo[m];                     // SUMASS (m=0) or CM(m)
-                const double ms = m == 0 ? 0.0 : mo[10 + m];  // SM(m)
-                const double ti = a.tin[(size_t)pos * kMaxTerms + t];
-                const double to = a.tout[(size_t)pos * kMaxTerms + t];
-                ic = fma(ti, mc, ic);
-                is = fma(ti, ms, is);
-                oc = fma(to, mc, oc);
-                os = fma(to, ms, os);
-            }
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int pos = p0 + grp + CHUNK_GROUPS * it;
+            // rings beyond REDGE are skipped (boundary.f:379)
+            const bool use = t < a.nt && pos < p1 && !(a.redge > 0.0 && rs[it] > a.redge);
+            const double s = m == 0 ? 0.0 : ms[it];
+            ic = use ? fma(ti[it], mc[it], ic) : ic;
+            is = use ? fma(ti[it], s, is) : is;
+            oc = use ? fma(to[it], mc[it], oc) : oc;
+            os = use ? fma(to[it], s, os) : os;
         }
         part[grp][t][0] = ic;
         part[grp][t][1] = is;
@@ -87,41 +99,43 @@ __global__ __launch_bounds__(256) void k_bdy_chunks(BdyArgs a)
 #pragma unroll
         for (int grp = 0; grp < CHUNK_GROUPS; ++grp) s += part[grp][t][v];
         const int comp = v >> 1, cs = v & 1;
-        a.chunk[((size_t)c * 2 + comp) * NCOL + 2 * t + cs] = s;
+        a.chunk[((size_t)comp * NCOL + 2 * t + cs) * a.nchunk + c] = s;
     }
 }
 
 // massio[c][0][col] = sum_{c'<c} chunk[c'][0][col]     (c = 0..nchunk)
 // massio[c][1][col] = sum_{c'>=c} chunk[c'][1][col]
-// one block per (column, component); Hillis-Steele scans of 256 in LDS
+// One block per (column, component) reads its column (contiguous in the
+// [comp][col][chunk] layout) 256 chunks at a time: inclusive scans within
+// each wavefront by shuffles, then the wavefront totals in a fixed order.
 __global__ __launch_bounds__(256) void k_bdy_scan(BdyArgs a)
 {
-    __shared__ double buf[256];
+    __shared__ double wsum[4];
     const int col = blockIdx.x % NCOL, comp = blockIdx.x / NCOL;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int n = a.nchunk;
+    const double *src = a.chunk + ((size_t)comp * NCOL + col) * n;
     double carry = 0.0;
     for (int base = 0; base < n; base += 256) {
         const int i = base + tid;  // position in scan order
-        double v = 0.0;
-        if (i < n) {
-            const int c = comp == 0 ? i : n - 1 - i;
-            v = a.chunk[((size_t)c * 2 + comp) * NCOL + col];
+        const int c = comp == 0 ? i : n - 1 - i;
+        const double x = src[i < n ? c : 0];
+        double v = i < n ? x : 0.0;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double u = __shfl_up(v, off, 64);
+            if (lane >= off) v += u;
         }
-        buf[tid] = v;
+        if (lane == 63) wsum[w] = v;
         __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            const double add = tid >= off ? buf[tid - off] : 0.0;
-            __syncthreads();
-            buf[tid] += add;
-            __syncthreads();
-        }
-        const double incl = carry + buf[tid];
+        double pre = carry;
+        for (int x2 = 0; x2 < w; ++x2) pre += wsum[x2];
+        const double incl = pre + v;
         if (i < n) {
             if (comp == 0) a.massio[((size_t)(i + 1) * 2 + 0) * NCOL + col] = incl;  // prefix of c' <= i
             else a.massio[((size_t)(n - 1 - i) * 2 + 1) * NCOL + col] = incl;      // suffix from c = n-1-i
         }
-        carry += buf[255];
+        carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
         __syncthreads();
     }
     if (tid == 0) {

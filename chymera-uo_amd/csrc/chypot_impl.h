@@ -28,6 +28,8 @@ constexpr int kTermM[kMaxTerms] = {0, 1, 0, 2, 1, 3, 0, 2, 4, 1, 3, 5, 0, 2, 4, 
                                    5, 7, 0, 2, 4, 6, 8, 1, 3, 5, 7, 9, 0, 2, 4, 6, 8, 10};
 // Rings per boundary chunk (the unit of the deterministic radial reduction).
 constexpr int kChunkRings = 64;
+// Captured solve graphs kept per grid (one per distinct argument set).
+constexpr int kGraphCache = 8;
 
 struct Term {
     int l, m;
@@ -47,14 +49,15 @@ struct DeviceGrid {
     double *wf = nullptr;        // [NZ][PP]   Y -> eigen coordinates:  Yh[p] = sum_k wf[k][p] Y[k]
     double *wi = nullptr;        // [NZ][PP]   eigen coordinates -> X:  X[k] = sum_p wi[p][k] Z[p]
     double *lam = nullptr;       // [PP]       eigenvalues (padded entries 1)
-    // r operator (pot3.f:172-194,226-230): AM(1)=0, CM(M)=0, BM per azimuthal order m
-    double *am = nullptr;        // [M]
-    double *cm = nullptr;        // [M]
-    double *bm = nullptr;        // [(N+1)][M]
+    // r operator (pot3.f:172-194,226-230): AM(1)=0, CM(M)=0, BM per azimuthal
+    // order m; rows padded to JP (rows >= M: AM = CM = 0, BM = 1)
+    double *am = nullptr;        // [JP]
+    double *cm = nullptr;        // [JP]
+    double *bm = nullptr;        // [(N+1)][JP]
     // ZAXPHI abscissae (pot3.f:342-348)
     double *zx = nullptr;        // [11]
     // BDYGEN
-    int *ring_of_pos = nullptr;  // [nring]   sorted position -> ring index (jj-2)+(J-1)(kk-2)
+    int *ring_of_pos = nullptr;  // [nring]   radius-sorted position -> ring index (jj-2)+(J-1)(kk-2)
     double *tin = nullptr;       // [nring][nt]  BLM(theta) r^l at sorted position
     double *tout = nullptr;      // [nring][nt]  BLM(theta) r^-(l+1)  (= tin * r^-(2l+1))
     double *rsph = nullptr;      // [nring]   spherical radius at sorted position
@@ -69,8 +72,8 @@ struct DeviceGrid {
     // workspace
     double *spec = nullptr;      // [L][KMAX][JMAX]  azimuthal spectrum, lanes J=2..JMAX+1, K=2..KMAX+1
     double *yh = nullptr;        // [L][PP][JP]      z-eigen coordinates of the mode RHS / solution
-    double *mom = nullptr;       // [nring][kMoments]
-    double *chunk = nullptr;     // [nchunk][2][2*kMaxTerms]  inside / outside chunk sums
+    double *mom = nullptr;       // [kMoments][nring] by ring index (the interior lane index)
+    double *chunk = nullptr;     // [2][2*kMaxTerms][nchunk]  inside / outside chunk sums
     double *massio = nullptr;    // [nchunk+1][2][2*kMaxTerms] exclusive prefix (in) / suffix (out)
     double *dscal = nullptr;     // [4 + KMAX+2]: tmass, phichk, klocat, _, phac[K]
     // staging for the host-pointer entry points (allocated on first use)
@@ -102,6 +105,14 @@ struct chy_ctx {
     hipStream_t stream = nullptr;
     bool graphs = true;  // replay chy_solve from captured HIP graphs
     chy::Profiler prof;
+    // chy_solve_batch: worker streams (default priority), fork/join events,
+    // per-item diagnostics staging
+    int nworkers = 4;
+    std::vector<hipStream_t> workers;
+    std::vector<hipEvent_t> worker_ev;
+    hipEvent_t fork_ev = nullptr;
+    double *d_diag = nullptr, *h_diag = nullptr;
+    int diag_cap = 0;
 };
 
 namespace chy {
@@ -137,8 +148,14 @@ struct chy_grid {
     size_t device_bytes = 0;
     std::vector<void *> allocations;
     chy::DeviceGrid d;
-    hipGraphExec_t graph_exec = nullptr;  // last captured chy_solve
-    chy::SolveKey graph_key{};
+    // captured chy_solve graphs, most recently used last (at most kGraphCache);
+    // `done` is recorded after every launch so an entry can be retired safely
+    struct GraphEntry {
+        chy::SolveKey key;
+        hipGraphExec_t exec;
+        hipEvent_t done;
+    };
+    std::vector<GraphEntry> graphs;
 };
 
 namespace chy {
@@ -160,7 +177,8 @@ int launch_fft_forward(const chy_grid *g, const double *rho, const double *phi, 
                        int coef_mode, int lanes, bool moments, bool spectrum, hipStream_t s);
 int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s);
 int launch_mode_solve(const chy_grid *g, hipStream_t s, Profiler *prof);
-int launch_ghosts_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s);
+// POT3's ghost cells (ghosts=true) and ZAXPHI with PHAC/PHICHK/KLOCAT.
+int launch_ghosts_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, bool ghosts, hipStream_t s);
 // BDYGEN from the ring moments in d.mom: radial split sums, then either the
 // boundary values and ghosts in PHI, or (spectral) the boundary lanes'
 // azimuthal spectrum written straight into d.spec with their COEF entries.

@@ -80,7 +80,7 @@ struct FftArgs {
     double *phi_out;      // PHI (inverse output)
     double *spec;         // [L][K][J], or null (forward: no spectrum written)
     double *coef;         // COEF(JMAX+2,KMAX+2,LMAX/2,2) or null
-    double *mom;          // ring moments [nring][21] of interior lanes, or null
+    double *mom;          // ring moments [21][nring] of interior lanes, or null
     const double *dv;     // DV per ring (boundary.f:516)
     const double2 *twL;   // exp(2 pi i e/L)
     const double2 *tw64;  // exp(2 pi i e/64)
@@ -90,6 +90,7 @@ struct FftArgs {
     int nlanes;
     int coef_mode;
     double sf1, pi;
+    double ys;            // factor on the interior lanes' spectrum (4 pi G)
 };
 
 // lane index -> (j,k), j = J-2 (0..JMAX-1), k = K-2 (0..KMAX-1)
@@ -189,24 +190,28 @@ __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
             if (r > N) X.y = -X.y;
             const double2 w = a.twh[mm];
             const double re = X.x * w.x - X.y * w.y, im = X.y * w.x + X.x * w.y;
-            double *o = a.mom + (size_t)ir * kMoments;
+            // component c of ring ir at mom[c][ir]: consecutive lanes, consecutive addresses
+            const size_t nr = (size_t)(a.J - 1) * (a.K - 1);
+            double *o = a.mom + ir;
             if (mm == 0) {
                 o[0] = dv * re;
             } else {
-                o[mm] = dv * re;
-                o[10 + mm] = dv * im;
+                o[mm * nr] = dv * re;
+                o[(10 + mm) * nr] = dv * im;
             }
         }
     }
     if (!a.spec) return;
     const bool want_coef = a.coef && (a.coef_mode == CHY_COEF_FULL || (a.coef_mode == CHY_COEF_K2 && k == 0));
+    // interior lanes feed the mode right-hand side Y = 4 pi G PHI~ (pot3.f:212)
+    const double ys = (j <= a.J - 2 && k <= a.K - 2) ? a.ys : 1.0;
     double A0 = 0.0;
     if (want_coef) A0 = 2.0 * xm(0).x;
     for (int m = sub; m <= N; m += N2) {
         const double2 X = xm(m);
         const double A = 2.0 * X.x, B = 2.0 * X.y;
-        a.spec[(size_t)m * JK + sidx] = A * a.sf1;
-        if (m >= 1 && m <= N - 1) a.spec[(size_t)(N + m) * JK + sidx] = B * a.sf1;
+        a.spec[(size_t)m * JK + sidx] = A * a.sf1 * ys;
+        if (m >= 1 && m <= N - 1) a.spec[(size_t)(N + m) * JK + sidx] = B * a.sf1 * ys;
         if (want_coef && m >= 1) {
             // COEF(J,K,m,1) amplitude, COEF(J,K,m,2) phase (pot3.f:105-132)
             const double x1 = fabs(A0);
@@ -371,6 +376,7 @@ FftArgs make_args(const chy_grid *g, int lanes)
              : lanes == LANES_INTERIOR ? (g->J - 1) * (g->K - 1) : g->J + g->K - 1;
     a.sf1 = 0.5 / (double)g->N;
     a.pi = g->pi;
+    a.ys = g->pig4;
     return a;
 }
 

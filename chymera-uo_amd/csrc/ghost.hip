@@ -22,6 +22,7 @@ struct GhostArgs {
     double *dscal;     // [0]=tmass [1]=phichk [2]=klocat [4+K]=PHAC(K)
     int J, K, L, JS, KS, npoint;
     int klocat_in;     // KLOCAT before the call (kept when every PHAC is 0)
+    int ghosts;        // also write the axis ghosts (POT3); 0 for ZAXPHI alone
 };
 
 // argmax of |v| with the smallest index winning ties, across one block
@@ -45,6 +46,7 @@ __device__ void block_argmax_abs(double v, int idx, double *sv, int *si, double 
     __syncthreads();
 }
 
+template <int NP>
 __global__ __launch_bounds__(256) void k_axis_zaxphi(GhostArgs a)
 {
     __shared__ double sv[256];
@@ -56,16 +58,19 @@ __global__ __launch_bounds__(256) void k_axis_zaxphi(GhostArgs a)
     const size_t row = (size_t)a.JS * (K - 1);
     double *phi = a.phi;
 
-    // axis ghost J=1
-    for (int l = tid; l < a.L; l += blockDim.x) {
-        const int lp = l < N ? l + N : l - N;
-        phi[row + plane * l] = phi[row + 1 + plane * lp];
+    // axis ghost J=1 (pot3.f:287-290)
+    if (a.ghosts) {
+        for (int l = tid; l < a.L; l += blockDim.x) {
+            const int lp = l < N ? l + N : l - N;
+            phi[row + plane * l] = phi[row + 1 + plane * lp];
+        }
     }
 
     // ZAXPHI abscissae (pot3.f:336-348)
-    const int np = a.npoint;
-    const int ichk = np % 2, nn = np - ichk, num = nn / 2, ispecl = num + num + 1, jsp = num + 1;
+    constexpr int np = NP;
+    constexpr int ichk = np % 2, nn = np - ichk, num = nn / 2, ispecl = num + num + 1, jsp = num + 1;
     double x[11];
+#pragma unroll
     for (int jj = 2; jj <= jsp; ++jj) {
         x[num + (jj - 1)] = a.zx[jj - 1];
         x[num - (jj - 2)] = -a.zx[jj - 1];
@@ -77,13 +82,16 @@ __global__ __launch_bounds__(256) void k_axis_zaxphi(GhostArgs a)
     int besti = 0x7fffffff;
     for (int l = tid; l < N; l += blockDim.x) {
         double p2[11];
+#pragma unroll
         for (int jj = 2; jj <= jsp; ++jj) {
             p2[num + (jj - 1)] = phi[row + (jj - 1) + plane * l];
             p2[num - (jj - 2)] = phi[row + (jj - 1) + plane * (l + N)];
         }
         if (ichk == 1) p2[ispecl] = phi[row + (num + 1) + plane * l];
+#pragma unroll
         for (int i = 1; i <= np - 1; ++i) {
             const double p1 = p2[i], xinv = 1.0 / x[i], xr = 0.0 * xinv;
+#pragma unroll
             for (int jj = i + 1; jj <= np; ++jj) {
                 const double xratio = x[jj] * xinv, pb = p2[jj];
                 p2[jj] = (p1 * (xratio - xr) + pb * (xr - 1.0)) / (xratio - 1.0);
@@ -154,18 +162,33 @@ __global__ __launch_bounds__(256) void k_phichk(GhostArgs a)
 
 }  // namespace
 
-int launch_ghosts_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s)
+int launch_ghosts_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, bool ghosts, hipStream_t s)
 {
     GhostArgs a{};
+    a.ghosts = ghosts ? 1 : 0;
     a.klocat_in = klocat_in;
     a.phi = phi;
     a.zx = g->d.zx;
     a.dscal = g->d.dscal;
     a.J = g->J; a.K = g->K; a.L = g->L; a.JS = g->J + 2; a.KS = g->K + 2;
     a.npoint = npoint;
-    hipLaunchKernelGGL(k_axis_zaxphi, dim3(g->K), dim3(256), 0, s, a);
-    const long total = (long)(g->J + 1) * g->L;
-    hipLaunchKernelGGL(k_midplane, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+    switch (npoint) {  // 1..10 (clamped by the callers)
+    case 1: hipLaunchKernelGGL(k_axis_zaxphi<1>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_axis_zaxphi<2>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_axis_zaxphi<3>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_axis_zaxphi<4>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 5: hipLaunchKernelGGL(k_axis_zaxphi<5>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 6: hipLaunchKernelGGL(k_axis_zaxphi<6>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 7: hipLaunchKernelGGL(k_axis_zaxphi<7>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_axis_zaxphi<8>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 9: hipLaunchKernelGGL(k_axis_zaxphi<9>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 10: hipLaunchKernelGGL(k_axis_zaxphi<10>, dim3(g->K), dim3(256), 0, s, a); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    if (ghosts) {
+        const long total = (long)(g->J + 1) * g->L;
+        hipLaunchKernelGGL(k_midplane, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+    }
     hipLaunchKernelGGL(k_phichk, dim3(1), dim3(256), 0, s, a);
     return (int)hipGetLastError();
 }

@@ -12,14 +12,15 @@
 // it is diagonalised once per grid (host, chy::build_grid), which turns each
 // mode's system into NZ independent r-tridiagonal systems
 //   (T_m + lambda_p) z_p = yhat_p,   yhat = Y W_f,   X = Z W_i^T.
-// Three kernels:
-//   k_zfwd    yhat[q][p][j] = sum_k W_f[k][p] Y_q[k][j]      (GEMM per mode)
-//   k_rsolve  one wavefront per (m, p): both the cosine and sine right-hand
-//             sides of order m, partitioned elimination in registers (each lane
-//             owns R consecutive rows) + parallel cyclic reduction across the
-//             64 lanes with cross-lane shuffles
-//   k_zinv    X_q[k][j] = sum_p W_i[k][p] z[q][p][j], written back into the
-//             spectrum (GEMM per mode)
+// Kernels:
+//   k_rhs_edges  the boundary terms of Y on the K=KMAX row and J=JMAX column
+//   z forward    yhat[q][p][j] = sum_k W_f[k][p] Y_q[k][j]      (GEMM per mode)
+//   r solves     one wavefront per (m, p): both the cosine and sine right-hand
+//                sides of order m, partitioned elimination in registers (each
+//                lane owns R consecutive rows) + parallel cyclic reduction
+//                across the 64 lanes with cross-lane shuffles
+//   z inverse    X_q[k][j] = sum_p W_i[k][p] z[q][p][j], written back into the
+//                spectrum (GEMM per mode)
 #include <algorithm>
 
 #include "chypot_impl.h"
@@ -87,13 +88,7 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void k_zgemm(ModeArgs a)
             const int k = k0 + kk, j = c0 + cc;
             double y = 0.0;
             if (k < nred && j < a.M) {
-                if (FWD) {  // Y of pot3.f:212-224
-                    y = a.pig4 * S[(size_t)k * a.J + j];
-                    if (k == a.NZ - 1) y -= a.cnmax * S[(size_t)a.NZ * a.J + j];
-                    if (j == a.M - 1) y -= a.cmmax * S[(size_t)k * a.J + a.M];
-                } else {
-                    y = Z[(size_t)k * a.JP + j];
-                }
+                y = FWD ? S[(size_t)k * a.J + j] : Z[(size_t)k * a.JP + j];
             }
             rb[i] = y;
         }
@@ -191,23 +186,21 @@ __global__ __launch_bounds__(PPT * 4) void k_zgemm_reg(ModeArgs a, int col_tiles
         ar[ks] = (kr < nred && r < nrow) ? W[(size_t)kr * a.PP + r] : 0.0;
     }
 
-    // fetch() only issues loads (clamped to valid addresses); the right-hand
-    // side is assembled in stash(), after the tile's products, so no load
-    // result is waited for inside the fetch.
-    double rb[BPER], ro[FWD ? BPER : 1], rtop = 0.0;
+    // fetch() only issues loads (clamped to valid addresses); stash(), after
+    // the tile's products, masks and stores them, so no load result is
+    // waited for inside the fetch.
+    double rb[BPER];
     const int cc = tid % PCT;  // this thread's column within every tile
     auto fetch = [&](int t) {
         const int q = t / col_tiles, j = (t % col_tiles) * PCT + cc;
         const int jc = j < a.M ? j : a.M - 1;
         if (FWD) {
             const double *S = a.spec + (size_t)q * a.J * a.K;
-            rtop = S[(size_t)a.NZ * a.J + jc];
 #pragma unroll
             for (int i = 0; i < BPER; ++i) {
                 const int kr = tid / PCT + (NT / PCT) * i;
                 const int kc = kr < nred ? kr : nred - 1;
                 rb[i] = S[(size_t)kc * a.J + jc];
-                ro[i] = S[(size_t)kc * a.J + a.M];
             }
         } else {
             const double *Z = a.yh + (size_t)q * a.PP * a.JP;
@@ -224,13 +217,7 @@ __global__ __launch_bounds__(PPT * 4) void k_zgemm_reg(ModeArgs a, int col_tiles
 #pragma unroll
         for (int i = 0; i < BPER; ++i) {
             const int kr = tid / PCT + (NT / PCT) * i;
-            double y = rb[i];
-            if (FWD) {  // Y of pot3.f:212-224
-                y = a.pig4 * y;
-                y = kr == a.NZ - 1 ? y - a.cnmax * rtop : y;
-                y = j == a.M - 1 ? y - a.cmmax * ro[i] : y;
-            }
-            Bs[buf][kr][cc] = (kr < nred && j < a.M) ? y : 0.0;
+            Bs[buf][kr][cc] = (kr < nred && j < a.M) ? rb[i] : 0.0;
         }
     };
 
@@ -302,38 +289,53 @@ __device__ __forceinline__ Red shfl_down_red(const Red &r, int s, int lane)
     return o;
 }
 
-// One wavefront solves (T_m + lambda_p) x = y for the cosine (q=m) and sine
-// (q=N+m, 1<=m<=N-1) right-hand sides.  Lane i owns rows [iR, iR+R); rows >= M
-// are identity rows.
+// R consecutive doubles at p (16-byte aligned when R is even) in 16-byte loads.
 template <int R>
-__global__ __launch_bounds__(256) void k_rsolve(ModeArgs a)
+__device__ __forceinline__ void load_rows(const double *p, double (&d)[R])
 {
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int sys = blockIdx.x * 4 + wave;
-    if (sys >= (a.N + 1) * a.NZ) return;  // whole wavefront exits together
-    const int m = sys / a.NZ, p = sys % a.NZ;
-    const bool two = (m >= 1 && m <= a.N - 1);
-    double *y0 = a.yh + ((size_t)m * a.PP + p) * a.JP;
-    double *y1 = a.yh + ((size_t)(a.N + m) * a.PP + p) * a.JP;
-    const double *bm = a.bm + (size_t)m * a.M;
+    if constexpr (R % 2 == 0) {
+        const double2 *p2 = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+        for (int i = 0; i < R / 2; ++i) {
+            const double2 v = p2[i];
+            d[2 * i] = v.x;
+            d[2 * i + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) d[i] = p[i];
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void store_rows(double *p, const double (&d)[R])
+{
+    if constexpr (R % 2 == 0) {
+        double2 *p2 = reinterpret_cast<double2 *>(p);
+#pragma unroll
+        for (int i = 0; i < R / 2; ++i) p2[i] = make_double2(d[2 * i], d[2 * i + 1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) p[i] = d[i];
+    }
+}
+
+// One wavefront solves (T_m + lambda_p) x = y for two right-hand sides d0, d1
+// (the cosine and sine parts of order m).  Lane i owns rows [iR, iR+R); rows
+// >= M are identity rows.  Inputs and outputs are the lane's R rows in
+// registers.
+template <int R>
+__device__ __forceinline__ void rsolve_rows(const ModeArgs &a, int m, int p, int lane, const double (&d0)[R],
+                                            const double (&d1)[R], double (&x0)[R], double (&x1)[R])
+{
     const double lam = a.lam[p];
     const int f = lane * R;
-
-    double ca[R], cb[R], cc[R], d0[R], d1[R];
+    double ca[R], cb[R], cc[R];
+    load_rows<R>(a.am + f, ca);
+    load_rows<R>(a.cm + f, cc);
+    load_rows<R>(a.bm + (size_t)m * a.JP + f, cb);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int j = f + r;
-        if (j < a.M) {
-            ca[r] = a.am[j];
-            cb[r] = bm[j] + lam;
-            cc[r] = a.cm[j];
-            d0[r] = y0[j];
-            d1[r] = two ? y1[j] : 0.0;
-        } else {
-            ca[r] = 0.0; cb[r] = 1.0; cc[r] = 0.0; d0[r] = 0.0; d1[r] = 0.0;
-        }
-    }
+    for (int r = 0; r < R; ++r) cb[r] = f + r < a.M ? cb[r] + lam : 1.0;  // padding rows: x = 0
 
     // local elimination of rows f..f+R-2 with the neighbouring interface
     // unknowns XL = x_{f-1} (last row of the previous lane) and XR = x_{f+R-1}
@@ -395,7 +397,6 @@ __global__ __launch_bounds__(256) void k_rsolve(ModeArgs a)
     const double xr0 = red.d0 / red.be, xr1 = red.d1 / red.be;
 
     // back-substitution into the lane's own rows
-    double x0[R], x1[R];
     x0[R - 1] = xr0;
     x1[R - 1] = xr1;
     if constexpr (R > 1) {
@@ -407,13 +408,57 @@ __global__ __launch_bounds__(256) void k_rsolve(ModeArgs a)
             x1[i] = D1[i] + P[i] * xl1 + Q[i] * xr1;
         }
     }
+}
+
+// Standalone r solves: one wavefront per (m, p) over the z-eigen coordinates
+// in yh, in place; the cosine (q=m) and sine (q=N+m, 1<=m<=N-1) right-hand
+// sides together.
+template <int R>
+__global__ __launch_bounds__(256) void k_rsolve(ModeArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int sys = blockIdx.x * 4 + wave;
+    if (sys >= (a.N + 1) * a.NZ) return;  // whole wavefront exits together
+    const int m = sys / a.NZ, p = sys % a.NZ;
+    const bool two = (m >= 1 && m <= a.N - 1);
+    double *y0 = a.yh + ((size_t)m * a.PP + p) * a.JP;
+    double *y1 = a.yh + ((size_t)(a.N + m) * a.PP + p) * a.JP;
+    const int f = lane * R;
+    // rows j >= M of yh are zero (written so by the z forward transform)
+    double d0[R], d1[R], x0[R], x1[R];
+    load_rows<R>(y0 + f, d0);
+    if (two) {
+        load_rows<R>(y1 + f, d1);
+    } else {
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int j = f + r;
-        if (j < a.JP) {
-            y0[j] = j < a.M ? x0[r] : 0.0;
-            if (two) y1[j] = j < a.M ? x1[r] : 0.0;
-        }
+        for (int r = 0; r < R; ++r) d1[r] = 0.0;
+    }
+    rsolve_rows<R>(a, m, p, lane, d0, d1, x0, x1);
+    store_rows<R>(y0 + f, x0);
+    if (two) store_rows<R>(y1 + f, x1);
+}
+
+// Y edge terms of pot3.f:212-224 on the spectrum (whose interior lanes the
+// forward transform has already scaled by 4 pi G): the K=KMAX row loses
+// CNMAX times the top boundary lane, the J=JMAX column CMMAX times the outer
+// boundary lane.  One thread per (q, edge cell); the corner takes both.
+__global__ __launch_bounds__(256) void k_rhs_edges(ModeArgs a)
+{
+    const int ne = a.M + a.NZ - 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ne * a.L) return;
+    const int q = (int)(t / ne), e = (int)(t % ne);
+    double *S = a.spec + (size_t)q * a.J * a.K;
+    if (e < a.M) {
+        const size_t c = (size_t)(a.NZ - 1) * a.J + e;
+        double y = S[c];
+        y -= a.cnmax * S[(size_t)a.NZ * a.J + e];
+        if (e == a.M - 1) y -= a.cmmax * S[c + 1];
+        S[c] = y;
+    } else {
+        const size_t c = (size_t)(e - a.M) * a.J + (a.M - 1);
+        S[c] -= a.cmmax * S[c + 1];
     }
 }
 
@@ -458,13 +503,18 @@ void launch_zgemm(const chy_grid *g, const ModeArgs &a, hipStream_t s)
 int launch_mode_solve(const chy_grid *g, hipStream_t s, Profiler *prof)
 {
     ModeArgs a = make_args(g);
+    const int R = g->JP / 64;
     if (prof) prof->begin(CHY_STAGE_ZFWD, s);
+    {
+        const long n = (long)(g->M + g->NZ - 1) * g->L;
+        hipLaunchKernelGGL(k_rhs_edges, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    }
     launch_zgemm<true>(g, a, s);
     if (prof) prof->end(s);
     const int nsys = (g->N + 1) * g->NZ;
     const int blocks = (nsys + 3) / 4;
     if (prof) prof->begin(CHY_STAGE_RSOLVE, s);
-    switch (g->JP / 64) {
+    switch (R) {
     case 1: hipLaunchKernelGGL(k_rsolve<1>, dim3(blocks), dim3(256), 0, s, a); break;
     case 2: hipLaunchKernelGGL(k_rsolve<2>, dim3(blocks), dim3(256), 0, s, a); break;
     case 4: hipLaunchKernelGGL(k_rsolve<4>, dim3(blocks), dim3(256), 0, s, a); break;

@@ -435,9 +435,19 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
     CHY_TRY(upload(g, &d.wf, wf));
     CHY_TRY(upload(g, &d.wi, wi));
     CHY_TRY(upload(g, &d.lam, lam));
-    CHY_TRY(upload(g, &d.am, am));
-    CHY_TRY(upload(g, &d.cm, cm));
-    CHY_TRY(upload(g, &d.bm, bmt));
+    {
+        // r coefficients padded to JP rows per order (rows >= M: AM = CM = 0,
+        // BM = 1) so that the r solves read whole aligned rows
+        const int JP = g->JP;
+        std::vector<double> amp(JP, 0.0), cmp(JP, 0.0), bmp((size_t)(N + 1) * JP, 1.0);
+        std::copy(am.begin(), am.end(), amp.begin());
+        std::copy(cm.begin(), cm.end(), cmp.begin());
+        for (int m = 0; m <= N; ++m)
+            std::copy(bmt.begin() + (size_t)m * M, bmt.begin() + (size_t)(m + 1) * M, bmp.begin() + (size_t)m * JP);
+        CHY_TRY(upload(g, &d.am, amp));
+        CHY_TRY(upload(g, &d.cm, cmp));
+        CHY_TRY(upload(g, &d.bm, bmp));
+    }
     CHY_TRY(upload(g, &d.zx, zx));
     CHY_TRY(upload(g, &d.ring_of_pos, order));
     CHY_TRY(upload(g, &d.tin, tin));

@@ -134,6 +134,30 @@ int chy_solve(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const
 /* Enable (default) or disable graph replay of chy_solve on this context. */
 int chy_ctx_graphs(chy_ctx *ctx, int enable);
 
+/* ---- batched solves (no reference counterpart: the reference solves one
+ * field per call; this serves many independent fields) ----
+ * One item per field: chy_solve's arguments.  The items run on the context's
+ * worker streams, forked from and joined back to the context's stream, so
+ * the call is ordered after earlier work on that stream and later work waits
+ * for the whole batch.  Items on one grid run in order on one worker (they
+ * share the grid's workspace); items on different grids may overlap.  Each
+ * item's results equal those of chy_solve on the same arguments.  If any item
+ * has a diag pointer the call waits and fills every such diag.  Every item is
+ * checked before anything is launched. */
+typedef struct {
+    chy_grid *grid;
+    const double *rho_d;
+    double *phi_d, *coef_d;
+    int coef_mode, maxtrm, isym, npoint;
+    double redge;
+    chy_diag *diag;  /* host, may be NULL */
+} chy_solve_item;
+#define CHY_MAX_WORKERS 16
+int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n);
+/* Number of worker streams for chy_solve_batch (1..CHY_MAX_WORKERS, default 4;
+ * waits for the context's streams). */
+int chy_ctx_workers(chy_ctx *ctx, int n);
+
 /* Host-pointer forms used by the Fortran shims: rho_h/phi_h/coef_h are the
  * COMMON arrays.  They copy to the grid's device buffers, run, copy back and
  * wait.  phi_h is read (boundary values, untouched cells) and written. */
@@ -141,6 +165,18 @@ int chy_bdygen_host(chy_grid *g, int maxtrm, int isym, double redge, const doubl
                     double *phi_h, chy_diag *diag);
 int chy_pot3_host(chy_grid *g, int npoint, const double *rho_h, double *phi_h, double *coef_h,
                   int coef_mode, chy_diag *diag);
+
+/* ---- ZAXPHI alone (pot/pot3.f:311-399) ----
+ * The NPOINT-point extrapolation of PHI to the z axis from PHI(J=2..,K,L) and
+ * PHI(J=2..,K,L+N): writes PHI(JMAX+2,K,L) and PHI(JMAX+2,K,L+N) for
+ * K=2..KMAX+1 and PHAC/PHICHK/KLOCAT as POT3's own call does (POT3 already
+ * includes it).  npoint as for chy_pot3; diag as for chy_pot3. */
+int chy_zaxphi(chy_grid *g, int npoint, double *phi_d, chy_diag *diag);
+int chy_zaxphi_host(chy_grid *g, int npoint, double *phi_h, chy_diag *diag);
+/* PHAC(K) for K=2..KMAX+1 (pot3.f:380-387) from the last POT3/ZAXPHI on the
+ * grid, n <= KMAX values into phac_h (host); waits for the stream.  For the
+ * IPRINT=1 report of pot3.f:402-404. */
+int chy_phac(chy_grid *g, double *phac_h, int n);
 
 /* ---- stage timing (no reference counterpart; for benchmarks) ----
  * While enabled, every stage of chy_bdygen/chy_pot3/chy_solve is bracketed by

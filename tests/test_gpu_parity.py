@@ -240,3 +240,51 @@ def test_solve_graph_replays_match_eager(ctx):
         ctx.graphs(True)
         pg.close()
     assert rel_max(chypot.to_host(eager["a"][0]), f["phi"]) < 1e-12
+
+
+def test_128x64x128_against_reference(ctx):
+    """The 128x64x128 spiral solve against the reference's planes (K=2 and
+    L=1), its boundary values, TMASS and COEF(K=2)."""
+    f = np.load(GOLDEN / "solve_large_128x64x128.npz")
+    g = fields.unit_grid(128, 64, 128)
+    rho = fields.density_spiral_torus(g, seed=2)
+    res = run_solve(ctx, g, rho, dict(isym=2, maxtrm=10, npoint=8, redge=0.0), coef_mode=chypot.COEF_K2)
+    phi, phib = res["phi"], res["phi_b"]
+    scale = float(f["phi_absmax"])
+    err = max(rel_max(phi[:, 1, :], f["phi_k2"], scale), rel_max(phi[:, :, 0], f["phi_l1"], scale))
+    print(f"[gpu] 128x64x128: vs reference {err:.3e}", flush=True)
+    assert err < 1e-12
+    assert rel_max(phib[1:g.jmax + 1, g.kmax, :], f["phi_b_top"]) < 1e-13
+    assert rel_max(phib[g.jmax, 1:g.kmax, :], f["phi_b_out"]) < 1e-13
+    assert abs(res["tmass"] - float(f["tmass"])) <= 1e-13 * abs(float(f["tmass"]))
+    assert np.abs(res["coef"][:, 1, :, 0] - f["coef_k2"][..., 0]).max() < 1e-11
+
+
+def test_512x256x512_discrete_operator(ctx):
+    """512x256x512 (KMAX=256: the general z GEMM, eight rows per lane in the r
+    solve, the 512-point transform): the solution satisfies the discrete
+    7-point operator of pot3.f:158-231 to its rounding floor, and TMASS is the
+    mass the oracle integrates."""
+    import pyoracle as po
+    po.use_extended(False)
+    g = fields.unit_grid(512, 256, 512)
+    rho = fields.density_spiral_torus(g, seed=2)
+    print("[gpu] 512x256x512 solve", flush=True)
+    pg = chypot.PreparedGrid(ctx, g)
+    rho_d = chypot.to_device(rho)
+    phi_d = torch.zeros_like(rho_d)
+    d = pg.solve(rho_d, phi_d, diag=True)
+    ctx.sync()
+    phi = chypot.to_host(phi_d)
+    del rho_d, phi_d
+    pg.close()
+    res, scale = po.residual(g, phi, rho)
+    inner = (slice(1, g.jmax), slice(1, g.kmax), slice(None))
+    worst = float(np.abs(res[inner] / scale[inner]).max())
+    print(f"[gpu] 512x256x512: max |residual|/scale {worst:.2e}", flush=True)
+    assert worst < 5e-11
+    # TMASS: the rings' mass, sum of rho DV (boundary.f:516,554)
+    rhf, r = g.rhf(), g.r()
+    dv = 0.5 * g.dtheta * g.delz * (r[2:g.jmax + 1] ** 2 - r[1:g.jmax] ** 2) * 2.0
+    mass = float(np.einsum("jkl,j->", rho[1:g.jmax, 1:g.kmax, :], dv))
+    assert abs(d.tmass - mass) <= 1e-12 * abs(mass)
