@@ -1,16 +1,27 @@
 #!/usr/bin/env python3
 """Potential-solve benchmark on one MI355X.
 
-A step is one potential solve -- BDYGEN + POT3 on one density field, device
-resident, COEF K=2 plane -- of the 256x128x256 spiral-arm torus (JMAX=256,
-KMAX=128, LMAX=256, DELR=DELZ=1/256, seeded).  `value` is solves per second over
-the timed steps with the density already in HBM.  The line also carries the
-per-step p50/p99 latency, the error against the fixture solutions (reference
-FP64 and the 80-bit solve of the same discrete system), per-stage times from
-HIP events on the solve's stream, the roofline of the dominant stage and the
-reference CPU solve timed on this host.
+Workload: the 256x128x256 spiral-arm torus (JMAX=256, KMAX=128, LMAX=256,
+DELR=DELZ=1/256, seeded), plain FP64.  One solve is BDYGEN + POT3 on one
+density field, device resident, COEF for the K=2 plane.  A step is one batch
+of B independent fields (different seeds, each on its own prepared grid)
+solved by one chy_solve_batch call on W worker streams; `value` is solves per
+second over the timed steps (B solves per step) with every density already in
+HBM.  B=1 is the single-field solve.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+The line also carries:
+  * p50/p99 latency of one solve (a separate pass of single-field chy_solve
+    calls timed with HIP events on the solve's stream) and of one batch step;
+  * the error of field 0 against the fixture solutions (reference FP64 and the
+    80-bit solve of the same discrete system);
+  * per-stage times and the roofline of the dominant kernel, from a pass of
+    single-field solves launched kernel by kernel with HIP events around every
+    stage on the solve's stream; `roofline.traffic` is the HBM bytes per launch
+    that rocprofv3 FETCH_SIZE/WRITE_SIZE recorded for that kernel
+    (profiles/potsolve_256x128x256_hbm_traffic.txt), or null;
+  * the reference CPU solve timed on this host.
+
+    python bench.py [--steps K] [--warmup W] [--batch B] [--workers W] [--gpus 1]
 """
 from __future__ import annotations
 
@@ -50,6 +61,27 @@ def stage_work(g):
         "ghost": dict(bytes=None, flops=None),
         "fft_bdy": dict(bytes=None, flops=None),
     }
+
+
+# stage -> kernel whose rocprofv3 counters stand for it
+STAGE_KERNEL = {"zfwd": "k_zgemm_reg<true, 128>", "zinv": "k_zgemm_reg<false, 128>",
+                "fft_fwd": "k_fft_forward<16, 8, 32>", "fft_inv": "k_fft_inverse<16, 8, 32>",
+                "rsolve": "k_rsolve<4>"}
+
+
+def hbm_traffic(stage):
+    """HBM bytes per launch of the stage's kernel as rocprofv3 FETCH_SIZE +
+    WRITE_SIZE recorded them, each scaled by what the counter read for a
+    device-to-device copy of known size (profiles/..._hbm_traffic.txt)."""
+    f = ROOT / "profiles" / "potsolve_256x128x256_hbm_traffic.txt"
+    name = STAGE_KERNEL.get(stage)
+    if name is None or not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("kernel=") and name in line:
+            kv = dict(p.split("=", 1) for p in line.split("\t"))
+            return float(kv["traffic_bytes"])
+    return None
 
 
 def cpu_baseline(g, rho, nrep):
@@ -97,6 +129,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=8, help="independent fields per step")
+    ap.add_argument("--workers", type=int, default=0, help="worker streams (default min(batch, 8))")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the solves, copy a buffer of this many MiB device to device "
                          "three times (a kernel of known traffic for scaling HBM counters)")
@@ -112,19 +146,31 @@ def main():
     from chypot import fields
 
     g = fields.unit_grid(256, 128, 256)
+    B = max(1, args.batch)
+    W = args.workers or min(B, 8)
     rho = fields.density_spiral_torus(g, seed=2)
     ctx = chypot.Context(0)
     # one explicit (default-priority) stream for the solves and the timing events
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
     ctx.set_stream(stream)
-    pg = chypot.PreparedGrid(ctx, g)
-    rho_d = chypot.to_device(rho)
-    phi_d = torch.zeros_like(rho_d)
-    coef_d = torch.zeros((2, g.lmax // 2, g.kmax + 2, g.jmax + 2), dtype=torch.float64, device="cuda")
+    ctx.workers(W)
+    jobs = []
+    for b in range(B):
+        rho_b = rho if b == 0 else fields.density_spiral_torus(g, seed=2 + b)
+        jobs.append(dict(grid=chypot.PreparedGrid(ctx, g), rho=chypot.to_device(rho_b),
+                         phi=torch.zeros((g.lmax, g.kmax + 2, g.jmax + 2), dtype=torch.float64,
+                                         device="cuda"),
+                         coef=torch.zeros((2, g.lmax // 2, g.kmax + 2, g.jmax + 2), dtype=torch.float64,
+                                          device="cuda"),
+                         coef_mode=chypot.COEF_K2))
+    j0 = jobs[0]
 
     def step():
-        pg.solve(rho_d, phi_d, coef=coef_d, coef_mode=chypot.COEF_K2)
+        ctx.solve_batch(jobs)
+
+    def single():
+        j0["grid"].solve(j0["rho"], j0["phi"], coef=j0["coef"], coef_mode=chypot.COEF_K2)
 
     if args.copy_mib > 0:
         src = torch.ones(args.copy_mib << 17, dtype=torch.float64, device="cuda")
@@ -136,29 +182,34 @@ def main():
 
     for _ in range(args.warmup):
         step()
+        single()
     torch.cuda.synchronize()
 
-    # timed region: graph replays of the solve (stage timing off)
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        step()
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    lat = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
-    elapsed = t1 - t0
-    value = args.steps / elapsed
-    phi = chypot.to_host(phi_d)
+    def timed(fn, n):
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(n):
+            starts[i].record(stream)
+            fn()
+            ends[i].record(stream)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        return t1 - t0, np.array([a.elapsed_time(b) for a, b in zip(starts, ends)])
 
-    # the same steps launched kernel by kernel with HIP events around every
+    # timed region: batch steps (graph replays on the worker streams)
+    elapsed, lat_batch = timed(step, args.steps)
+    value = B * args.steps / elapsed
+    phi = chypot.to_host(j0["phi"])
+    # single-solve latency: one field per call on the context's stream
+    _, lat = timed(single, args.steps)
+
+    # single-field steps launched kernel by kernel with HIP events around every
     # stage on the solve's stream: per-kernel durations for the roofline
     ctx.profile(True)
     for _ in range(args.steps):
-        step()
+        single()
     stages = ctx.stage_times()
     ctx.profile(False)
 
@@ -188,6 +239,7 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "avg_ms": avg_ms,
                 "launches": stages[dom][1]}
 
+    roof["traffic"] = hbm_traffic(dom)
     cpu = None if args.no_cpu_baseline else cpu_baseline(g, rho, args.cpu_reps)
     out = {
         "metric": METRIC,
@@ -203,10 +255,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "plain FP64 potential solve (BDYGEN+POT3), spiral-arm torus",
-                   "grid": g.name, "path": "plain_fp64", "coef": "K=2 plane",
-                   "maxtrm": 10, "isym": 2, "npoint": 8},
+                   "grid": g.name, "path": "plain_fp64", "batch": B, "workers": W,
+                   "coef": "K=2 plane", "maxtrm": 10, "isym": 2, "npoint": 8},
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
+        "single_solves_per_s": float(1e3 / np.mean(lat)),
+        "batch_step_p50_ms": float(np.percentile(lat_batch, 50)),
+        "batch_step_p99_ms": float(np.percentile(lat_batch, 99)),
         "max_rel_err_vs_reference_fp64": e_ref,
         "max_rel_err_vs_exact": e_ext,
         "stages_ms": {k: round(v, 5) for k, v in stage_avg.items()},

@@ -163,7 +163,7 @@ static int run_pot3_tail(chy_grid *g, hipStream_t s, int npoint, double *phi, in
     if (p) p->end(s);
     if (rc) return rc;
     if (p) p->begin(CHY_STAGE_GHOST, s);
-    rc = dev_rc(launch_ghosts_zaxphi(g, npoint, klocat_in, phi, true, s), "ghost/ZAXPHI launch");
+    rc = dev_rc(launch_zaxphi(g, npoint, klocat_in, phi, s), "ZAXPHI launch");
     if (p) p->end(s);
     return rc;
 }
@@ -629,7 +629,7 @@ int chy_zaxphi(chy_grid *g, int npoint, double *phi_d, chy_diag *diag)
     hipStream_t s = g->ctx->stream;
     Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
     if (p) p->begin(CHY_STAGE_GHOST, s);
-    rc = dev_rc(launch_ghosts_zaxphi(g, npoint, diag ? diag->klocat : 0, phi_d, false, s), "ZAXPHI launch");
+    rc = dev_rc(launch_zaxphi(g, npoint, diag ? diag->klocat : 0, phi_d, s), "ZAXPHI launch");
     if (p) p->end(s);
     if (rc == CHY_OK && diag) rc = read_diag(g, diag, false, true);
     return rc;
@@ -648,7 +648,7 @@ int chy_zaxphi_host(chy_grid *g, int npoint, double *phi_h, chy_diag *diag)
     const size_t fb = sizeof(double) * field_elems(g);
     hipError_t e = hipMemcpyAsync(g->d.h_phi, phi_h, fb, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return dev_rc((int)e, "host to device copy");
-    rc = dev_rc(launch_ghosts_zaxphi(g, npoint, diag ? diag->klocat : 0, g->d.h_phi, false, s), "ZAXPHI launch");
+    rc = dev_rc(launch_zaxphi(g, npoint, diag ? diag->klocat : 0, g->d.h_phi, s), "ZAXPHI launch");
     if (rc) return rc;
     e = hipMemcpyAsync(phi_h, g->d.h_phi, fb, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);

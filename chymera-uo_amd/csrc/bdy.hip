@@ -34,7 +34,7 @@ constexpr int CHUNK_GROUPS = 7;      // ring groups per chunk block (7 x 36 <= 2
 struct BdyArgs {
     double *phi;
     const double *cosm, *sinm;
-    const double *mom;            // [kMoments][nring] by ring index
+    const double *mom;            // [nring][kMoments] by ring index
     const int *ring_of_pos;       // radius-sorted position -> ring index
     const double *tin, *tout, *rsph;
     const int *chunk_start;
@@ -54,38 +54,45 @@ struct BdyArgs {
 __global__ __launch_bounds__(256) void k_bdy_chunks(BdyArgs a)
 {
     constexpr int IT = (kChunkRings + CHUNK_GROUPS - 1) / CHUNK_GROUPS;  // rings per thread
+    __shared__ double smom[kChunkRings][kMoments + 1];
     __shared__ double part[CHUNK_GROUPS][kMaxTerms][4];
     const int c = blockIdx.x;
     const int tid = threadIdx.x;
     const int p0 = a.chunk_start[c], p1 = a.chunk_start[c + 1];
-    if (tid < CHUNK_GROUPS * kMaxTerms) {
-        const int t = tid % kMaxTerms, grp = tid / kMaxTerms;
-        const int m = kTermM[t];
-        // every load first (positions clamped into the chunk), then the sums
-        // in ascending position order
-        double mc[IT], ms[IT], ti[IT], to[IT], rs[IT];
+    const int t = tid % kMaxTerms, grp = tid / kMaxTerms;
+    const bool worker = tid < CHUNK_GROUPS * kMaxTerms;
+    // this thread's ring terms first (positions clamped into the chunk) ...
+    double ti[IT], to[IT], rs[IT];
 #pragma unroll
-        for (int it = 0; it < IT; ++it) {
-            const int pos = p0 + grp + CHUNK_GROUPS * it;
-            const int pc = pos < p1 ? pos : p0;
-            const double *mo = a.mom + a.ring_of_pos[pc];
-            mc[it] = mo[(size_t)m * a.nring];         // SUMASS (m=0) or CM(m)
-            ms[it] = mo[(size_t)(10 + m) * a.nring];  // SM(m) for m >= 1
-            ti[it] = a.tin[(size_t)pc * kMaxTerms + t];
-            to[it] = a.tout[(size_t)pc * kMaxTerms + t];
-            rs[it] = a.rsph[pc];
-        }
+    for (int it = 0; it < IT; ++it) {
+        const int pos = p0 + grp + CHUNK_GROUPS * it;
+        const int pc = (worker && pos < p1) ? pos : p0;
+        ti[it] = a.tin[(size_t)pc * kMaxTerms + t];
+        to[it] = a.tout[(size_t)pc * kMaxTerms + t];
+        rs[it] = a.rsph[pc];
+    }
+    // ... while the block gathers its rings' moments (ring order in memory,
+    // radius order here) into LDS
+    for (int v = tid; v < (p1 - p0) * kMoments; v += blockDim.x) {
+        const int r = v / kMoments, cc = v % kMoments;
+        smom[r][cc] = a.mom[(size_t)a.ring_of_pos[p0 + r] * kMoments + cc];
+    }
+    __syncthreads();
+    if (worker) {
+        const int m = kTermM[t];
         double ic = 0.0, is = 0.0, oc = 0.0, os = 0.0;
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
-            const int pos = p0 + grp + CHUNK_GROUPS * it;
+            const int r = grp + CHUNK_GROUPS * it;
             // rings beyond REDGE are skipped (boundary.f:379)
-            const bool use = t < a.nt && pos < p1 && !(a.redge > 0.0 && rs[it] > a.redge);
-            const double s = m == 0 ? 0.0 : ms[it];
-            ic = use ? fma(ti[it], mc[it], ic) : ic;
-            is = use ? fma(ti[it], s, is) : is;
-            oc = use ? fma(to[it], mc[it], oc) : oc;
-            os = use ? fma(to[it], s, os) : os;
+            const bool use = t < a.nt && p0 + r < p1 && !(a.redge > 0.0 && rs[it] > a.redge);
+            const int rr = use ? r : 0;
+            const double mc = smom[rr][m];                       // SUMASS (m=0) or CM(m)
+            const double ms = m == 0 ? 0.0 : smom[rr][10 + m];   // SM(m)
+            ic = use ? fma(ti[it], mc, ic) : ic;
+            is = use ? fma(ti[it], ms, is) : is;
+            oc = use ? fma(to[it], mc, oc) : oc;
+            os = use ? fma(to[it], ms, os) : os;
         }
         part[grp][t][0] = ic;
         part[grp][t][1] = is;
@@ -94,12 +101,12 @@ __global__ __launch_bounds__(256) void k_bdy_chunks(BdyArgs a)
     }
     __syncthreads();
     if (tid < kMaxTerms * 4) {
-        const int t = tid / 4, v = tid % 4;
-        double s = 0.0;
+        const int tt = tid / 4, v = tid % 4;
+        double sum = 0.0;
 #pragma unroll
-        for (int grp = 0; grp < CHUNK_GROUPS; ++grp) s += part[grp][t][v];
+        for (int g2 = 0; g2 < CHUNK_GROUPS; ++g2) sum += part[g2][tt][v];
         const int comp = v >> 1, cs = v & 1;
-        a.chunk[((size_t)comp * NCOL + 2 * t + cs) * a.nchunk + c] = s;
+        a.chunk[((size_t)comp * NCOL + 2 * tt + cs) * a.nchunk + c] = sum;
     }
 }
 

@@ -72,10 +72,11 @@ struct DeviceGrid {
     // workspace
     double *spec = nullptr;      // [L][KMAX][JMAX]  azimuthal spectrum, lanes J=2..JMAX+1, K=2..KMAX+1
     double *yh = nullptr;        // [L][PP][JP]      z-eigen coordinates of the mode RHS / solution
-    double *mom = nullptr;       // [kMoments][nring] by ring index (the interior lane index)
+    double *mom = nullptr;       // [nring][kMoments] by ring index (the interior lane index)
     double *chunk = nullptr;     // [2][2*kMaxTerms][nchunk]  inside / outside chunk sums
     double *massio = nullptr;    // [nchunk+1][2][2*kMaxTerms] exclusive prefix (in) / suffix (out)
     double *dscal = nullptr;     // [4 + KMAX+2]: tmass, phichk, klocat, _, phac[K]
+    unsigned *ticket = nullptr;  // ZAXPHI blocks finished (0 between launches)
     // staging for the host-pointer entry points (allocated on first use)
     double *h_rho = nullptr, *h_phi = nullptr, *h_coef = nullptr;
 };
@@ -177,8 +178,8 @@ int launch_fft_forward(const chy_grid *g, const double *rho, const double *phi, 
                        int coef_mode, int lanes, bool moments, bool spectrum, hipStream_t s);
 int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s);
 int launch_mode_solve(const chy_grid *g, hipStream_t s, Profiler *prof);
-// POT3's ghost cells (ghosts=true) and ZAXPHI with PHAC/PHICHK/KLOCAT.
-int launch_ghosts_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, bool ghosts, hipStream_t s);
+// ZAXPHI with PHAC/PHICHK/KLOCAT (POT3's ghost cells come with the inverse FFT).
+int launch_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s);
 // BDYGEN from the ring moments in d.mom: radial split sums, then either the
 // boundary values and ghosts in PHI, or (spectral) the boundary lanes'
 // azimuthal spectrum written straight into d.spec with their COEF entries.

@@ -9,7 +9,8 @@
 //     b_m = (2/L) sum_n x_n sin(2 pi m n / L),  m = 1..N-1
 // The COEF diagnostics of pot/pot3.f:105-132 are formed from the same
 // coefficients.  Inverse (pot/pot3.f:256-283): the real Fourier series
-//     x_n = a_0/2 + sum_{m=1}^{N-1} (a_m cos + b_m sin)(2 pi m n/L) + a_N/2 (-1)^n.
+//     x_n = a_0/2 + sum_{m=1}^{N-1} (a_m cos + b_m sin)(2 pi m n/L) + a_N/2 (-1)^n,
+// written with POT3's axis and midplane ghost cells (pot/pot3.f:287-296).
 //
 // Layout.  Lanes are consecutive J (the contiguous index of PHI), so every
 // load and store of a wave touches contiguous J runs.  The spectrum goes to a
@@ -80,7 +81,7 @@ struct FftArgs {
     double *phi_out;      // PHI (inverse output)
     double *spec;         // [L][K][J], or null (forward: no spectrum written)
     double *coef;         // COEF(JMAX+2,KMAX+2,LMAX/2,2) or null
-    double *mom;          // ring moments [21][nring] of interior lanes, or null
+    double *mom;          // ring moments [nring][21] of interior lanes, or null
     const double *dv;     // DV per ring (boundary.f:516)
     const double2 *twL;   // exp(2 pi i e/L)
     const double2 *tw64;  // exp(2 pi i e/64)
@@ -190,14 +191,12 @@ __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
             if (r > N) X.y = -X.y;
             const double2 w = a.twh[mm];
             const double re = X.x * w.x - X.y * w.y, im = X.y * w.x + X.x * w.y;
-            // component c of ring ir at mom[c][ir]: consecutive lanes, consecutive addresses
-            const size_t nr = (size_t)(a.J - 1) * (a.K - 1);
-            double *o = a.mom + ir;
+            double *o = a.mom + (size_t)ir * kMoments;  // the lanes' records are contiguous
             if (mm == 0) {
                 o[0] = dv * re;
             } else {
-                o[mm * nr] = dv * re;
-                o[(10 + mm) * nr] = dv * im;
+                o[mm] = dv * re;
+                o[10 + mm] = dv * im;
             }
         }
     }
@@ -322,14 +321,27 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
     if (!active) return;
     const size_t plane = (size_t)a.JS * a.KS;
     const size_t base = (size_t)(j + 1) + (size_t)a.JS * (size_t)(k + 1);
+    // POT3's ghost cells from the same values (pot3.f:287-296): the J=2 lanes
+    // also write the axis ghost PHI(1,K,L)=PHI(2,K,L+-N), the K=2 lanes the
+    // midplane row PHI(J,1,L)=PHI(J,2,L), the (2,2) lane both and PHI(1,1,L)
+    const bool axis = j == 0, mid = k == 0;
+    auto put = [&](int l, double v) {
+        a.phi_out[base + plane * (size_t)l] = v;
+        if (mid) a.phi_out[base - a.JS + plane * (size_t)l] = v;
+        if (axis) {
+            const size_t lg = (size_t)((l + N) % L);
+            a.phi_out[base - 1 + plane * lg] = v;
+            if (mid) a.phi_out[base - 1 - a.JS + plane * lg] = v;
+        }
+    };
 #pragma unroll
     for (int r = 0; r < R2; ++r) {
         const int k1 = sub + N2 * r;
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) {
             const int n = k1 + N1 * k2;
-            a.phi_out[base + plane * (size_t)(2 * n)] = u[r][k2].x;
-            a.phi_out[base + plane * (size_t)(2 * n + 1)] = u[r][k2].y;
+            put(2 * n, u[r][k2].x);
+            put(2 * n + 1, u[r][k2].y);
         }
     }
 }

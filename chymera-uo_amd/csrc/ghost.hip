@@ -1,16 +1,14 @@
-// POT3 epilogue: ghost cells (pot/pot3.f:287-298) and ZAXPHI
-// (pot/pot3.f:311-399), after the inverse FFT has written J=2..JMAX+1,
-// K=2..KMAX+1.
-//   k_axis_zaxphi  one block per K=2..KMAX+1: the axis ghost
-//                  PHI(1,K,L)=PHI(2,K,L+-N) (the cell across the axis), the
-//                  NPOINT Aitken extrapolation of PHI to r=0 from
-//                  x=+-RHF(2..) written to PHI(JMAX+2,K,L) and
-//                  PHI(JMAX+2,K,L+N), and PHAC(K), the largest azimuthal
-//                  spread of that value (first index wins ties, as the
-//                  reference's sequential strict-greater scans do)
-//   k_midplane     PHI(J,1,L)=PHI(J,2,L) for J=1..JMAX+1 (the J=1 entry is
-//                  the axis ghost of K=2, taken at its source)
-//   k_phichk       PHICHK/KLOCAT: the PHAC of largest magnitude, first K wins
+// ZAXPHI (pot/pot3.f:311-399) after the inverse FFT has written PHI on
+// J=1..JMAX+1, K=1..KMAX+1 (its ghost cells included, fft.hip).
+//   k_zaxphi  one block per K=2..KMAX+1: the NPOINT Aitken extrapolation of
+//             PHI to r=0 from x=+-RHF(2..) written to PHI(JMAX+2,K,L) and
+//             PHI(JMAX+2,K,L+N), and PHAC(K), the largest azimuthal spread of
+//             that value (first index wins ties, as the reference's
+//             sequential strict-greater scans do).  The last block to finish
+//             (a ticket counter) takes PHICHK/KLOCAT over all PHAC: the PHAC
+//             of largest magnitude, first K wins, KLOCAT unchanged when every
+//             PHAC is 0 (pot3.f:391-399); it resets the counter for the next
+//             launch.
 #include "chypot_impl.h"
 
 namespace chy {
@@ -22,7 +20,7 @@ struct GhostArgs {
     double *dscal;     // [0]=tmass [1]=phichk [2]=klocat [4+K]=PHAC(K)
     int J, K, L, JS, KS, npoint;
     int klocat_in;     // KLOCAT before the call (kept when every PHAC is 0)
-    int ghosts;        // also write the axis ghosts (POT3); 0 for ZAXPHI alone
+    unsigned *ticket;  // blocks finished (0 between launches)
 };
 
 // argmax of |v| with the smallest index winning ties, across one block
@@ -47,7 +45,7 @@ __device__ void block_argmax_abs(double v, int idx, double *sv, int *si, double 
 }
 
 template <int NP>
-__global__ __launch_bounds__(256) void k_axis_zaxphi(GhostArgs a)
+__global__ __launch_bounds__(256) void k_zaxphi(GhostArgs a)
 {
     __shared__ double sv[256];
     __shared__ int si[256];
@@ -57,14 +55,6 @@ __global__ __launch_bounds__(256) void k_axis_zaxphi(GhostArgs a)
     const size_t plane = (size_t)a.JS * a.KS;
     const size_t row = (size_t)a.JS * (K - 1);
     double *phi = a.phi;
-
-    // axis ghost J=1 (pot3.f:287-290)
-    if (a.ghosts) {
-        for (int l = tid; l < a.L; l += blockDim.x) {
-            const int lp = l < N ? l + N : l - N;
-            phi[row + plane * l] = phi[row + 1 + plane * lp];
-        }
-    }
 
     // ZAXPHI abscissae (pot3.f:336-348)
     constexpr int np = NP;
@@ -117,79 +107,58 @@ __global__ __launch_bounds__(256) void k_axis_zaxphi(GhostArgs a)
     double e;
     int ei;
     block_argmax_abs(err, erri, sv, si, e, ei);
-    if (tid == 0) a.dscal[4 + K] = e;
-}
 
-__global__ __launch_bounds__(256) void k_midplane(GhostArgs a)
-{
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long total = (long)(a.J + 1) * a.L;
-    if (idx >= total) return;
-    const int jf = (int)(idx % (a.J + 1)) + 1;  // Fortran J = 1..JMAX+1
-    const int l = (int)(idx / (a.J + 1));
-    const int N = a.L / 2;
-    const size_t plane = (size_t)a.JS * a.KS;
-    double v;
-    if (jf == 1) {
-        const int lp = l < N ? l + N : l - N;
-        v = a.phi[(size_t)1 + (size_t)a.JS + plane * lp];  // PHI(2,2,L+-N)
-    } else {
-        v = a.phi[(size_t)(jf - 1) + (size_t)a.JS + plane * l];
+    // PHICHK/KLOCAT by the last block to finish
+    __shared__ bool last;
+    if (tid == 0) {
+        a.dscal[4 + K] = e;
+        __threadfence();
+        last = atomicAdd(a.ticket, 1u) == gridDim.x - 1;
     }
-    a.phi[(size_t)(jf - 1) + plane * l] = v;
-}
-
-// PHICHK = PHAC(K) of largest magnitude, KLOCAT its K; the first K wins ties and
-// KLOCAT keeps its old value when every PHAC is 0 (pot3.f:391-399)
-__global__ __launch_bounds__(256) void k_phichk(GhostArgs a)
-{
-    __shared__ double sv[256];
-    __shared__ int si[256];
-    double best = 0.0;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double best2 = 0.0;
     int bk = 0x7fffffff;
-    for (int K = 2 + threadIdx.x; K <= a.K + 1; K += blockDim.x) {
-        const double v = a.dscal[4 + K];
-        if (fabs(v) > fabs(best)) { best = v; bk = K; }
+    for (int kk = 2 + tid; kk <= a.K + 1; kk += blockDim.x) {
+        const double v = __hip_atomic_load(&a.dscal[4 + kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (fabs(v) > fabs(best2)) { best2 = v; bk = kk; }
     }
-    double e;
-    int ek;
-    block_argmax_abs(best, bk, sv, si, e, ek);
-    if (threadIdx.x == 0) {
-        a.dscal[1] = e;
-        a.dscal[2] = fabs(e) > 0.0 ? (double)ek : (double)a.klocat_in;
+    double pc;
+    int pk;
+    block_argmax_abs(best2, bk, sv, si, pc, pk);
+    if (tid == 0) {
+        a.dscal[1] = pc;
+        a.dscal[2] = fabs(pc) > 0.0 ? (double)pk : (double)a.klocat_in;
+        *a.ticket = 0u;
     }
 }
 
 }  // namespace
 
-int launch_ghosts_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, bool ghosts, hipStream_t s)
+int launch_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s)
 {
     GhostArgs a{};
-    a.ghosts = ghosts ? 1 : 0;
     a.klocat_in = klocat_in;
     a.phi = phi;
     a.zx = g->d.zx;
     a.dscal = g->d.dscal;
+    a.ticket = g->d.ticket;
     a.J = g->J; a.K = g->K; a.L = g->L; a.JS = g->J + 2; a.KS = g->K + 2;
     a.npoint = npoint;
     switch (npoint) {  // 1..10 (clamped by the callers)
-    case 1: hipLaunchKernelGGL(k_axis_zaxphi<1>, dim3(g->K), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_axis_zaxphi<2>, dim3(g->K), dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL(k_axis_zaxphi<3>, dim3(g->K), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(k_axis_zaxphi<4>, dim3(g->K), dim3(256), 0, s, a); break;
-    case 5: hipLaunchKernelGGL(k_axis_zaxphi<5>, dim3(g->K), dim3(256), 0, s, a); break;
-    case 6: hipLaunchKernelGGL(k_axis_zaxphi<6>, dim3(g->K), dim3(256), 0, s, a); break;
-    case 7: hipLaunchKernelGGL(k_axis_zaxphi<7>, dim3(g->K), dim3(256), 0, s, a); break;
-    case 8: hipLaunchKernelGGL(k_axis_zaxphi<8>, dim3(g->K), dim3(256), 0, s, a); break;
-    case 9: hipLaunchKernelGGL(k_axis_zaxphi<9>, dim3(g->K), dim3(256), 0, s, a); break;
-    case 10: hipLaunchKernelGGL(k_axis_zaxphi<10>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL(k_zaxphi<1>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_zaxphi<2>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_zaxphi<3>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_zaxphi<4>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 5: hipLaunchKernelGGL(k_zaxphi<5>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 6: hipLaunchKernelGGL(k_zaxphi<6>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 7: hipLaunchKernelGGL(k_zaxphi<7>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 8: hipLaunchKernelGGL(k_zaxphi<8>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 9: hipLaunchKernelGGL(k_zaxphi<9>, dim3(g->K), dim3(256), 0, s, a); break;
+    case 10: hipLaunchKernelGGL(k_zaxphi<10>, dim3(g->K), dim3(256), 0, s, a); break;
     default: return (int)hipErrorInvalidValue;
     }
-    if (ghosts) {
-        const long total = (long)(g->J + 1) * g->L;
-        hipLaunchKernelGGL(k_midplane, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
-    }
-    hipLaunchKernelGGL(k_phichk, dim3(1), dim3(256), 0, s, a);
     return (int)hipGetLastError();
 }
 
