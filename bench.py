@@ -64,7 +64,7 @@ def stage_work(g):
 
 
 # stage -> kernel whose rocprofv3 counters stand for it
-STAGE_KERNEL = {"zfwd": "k_zgemm_reg<true, 128>", "zinv": "k_zgemm_reg<false, 128>",
+STAGE_KERNEL = {"zfwd": "k_zgemm_reg<true, 128, 64, 1>", "zinv": "k_zgemm_reg<false, 128, 64, 1>",
                 "fft_fwd": "k_fft_forward<16, 8, 32>", "fft_inv": "k_fft_inverse<16, 8, 32>",
                 "rsolve": "k_rsolve<4>"}
 

@@ -153,10 +153,10 @@ static int run_bdygen(chy_grid *g, int maxtrm, int isym, double redge, const dou
 }
 
 // the solve after the forward transform: mode solves, inverse transform, ghosts, ZAXPHI
-static int run_pot3_tail(chy_grid *g, hipStream_t s, int npoint, double *phi, int klocat_in)
+static int run_pot3_tail(chy_grid *g, hipStream_t s, int npoint, double *phi, int klocat_in, bool rhs_edges)
 {
     Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
-    int rc = dev_rc(launch_mode_solve(g, s, p), "mode solve launch");
+    int rc = dev_rc(launch_mode_solve(g, rhs_edges, s, p), "mode solve launch");
     if (rc) return rc;
     if (p) p->begin(CHY_STAGE_FFT_INV, s);
     rc = dev_rc(launch_fft_inverse(g, phi, s), "inverse FFT launch");
@@ -194,7 +194,7 @@ static int run_solve_eager(chy_grid *g, hipStream_t s, int maxtrm, int isym, dou
         if (p) p->end(s);
         if (rc) return rc;
     }
-    return run_pot3_tail(g, s, npoint, phi, klocat_in);
+    return run_pot3_tail(g, s, npoint, phi, klocat_in, !spectral);
 }
 
 // The same sequence replayed from a HIP graph (captured on first use for a
@@ -256,7 +256,7 @@ static int run_pot3(chy_grid *g, int npoint, const double *rho, double *phi, dou
                     "forward FFT launch");
     if (p) p->end(s);
     if (rc) return rc;
-    return run_pot3_tail(g, s, npoint, phi, klocat_in);
+    return run_pot3_tail(g, s, npoint, phi, klocat_in, true);
 }
 
 static int read_diag(chy_grid *g, chy_diag *diag, bool tmass, bool zax)

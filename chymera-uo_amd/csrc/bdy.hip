@@ -46,6 +46,7 @@ struct BdyArgs {
     double *dscal;
     int J, K, L, JS, KS, nring, nchunk, jkmax, nt, isym;
     double redge, grav, pi;
+    double cnmax, cmmax;          // boundary coefficients of the mode RHS (pot3.f:184-191)
 };
 
 // chunk[0][2t+s][c]: sum over chunk c's rings of TERM_t * moment_s (inside
@@ -213,12 +214,10 @@ __global__ __launch_bounds__(256) void k_bdy_eval(BdyArgs a)
 //   b_m = -2 G (-C_m sin(m DTHETA/2) + S_m cos(m DTHETA/2))
 // and every other coefficient is 0.  Valid for LMAX/2 > MAXTRM.  Writes the
 // spectrum of the boundary lane and its COEF entries (pot3.f:105-132).
-__global__ __launch_bounds__(64) void k_bdy_spec(BdyArgs a, double *spec, const double2 *twh, double *coef,
-                                                 int coef_mode)
+// C_m, S_m (m = 0..10) of boundary point jk: the TRMTOT sums of
+// boundary.f:782-806 grouped by azimuthal order (block-cooperative, 64 threads)
+__device__ void point_sums(const BdyArgs &a, int jk, double *t1, double *t2, double *cmv, double *smv)
 {
-    __shared__ double t1[kMaxTerms], t2[kMaxTerms];
-    __shared__ double cmv[kMaxTrm + 1], smv[kMaxTrm + 1];
-    const int jk = blockIdx.x;
     const int tid = threadIdx.x;
     if (tid < kMaxTerms) {
         double v1 = 0.0, v2 = 0.0;
@@ -254,22 +253,60 @@ __global__ __launch_bounds__(64) void k_bdy_spec(BdyArgs a, double *spec, const 
         smv[tid] = s;
     }
     __syncthreads();
+}
+
+// packed coefficient q of the boundary series with sums cmv/smv
+__device__ __forceinline__ double packed_coef(const double *cmv, const double *smv, const double2 *twh, int q,
+                                              int N, double g2)
+{
+    if (q == 0) return g2 * cmv[0];
+    if (q <= kMaxTrm) return g2 * (cmv[q] * twh[q].x + smv[q] * twh[q].y);
+    if (q > N && q - N <= kMaxTrm) {
+        const int m = q - N;
+        return g2 * (-cmv[m] * twh[m].y + smv[m] * twh[m].x);
+    }
+    return 0.0;
+}
+
+// With `edges`, the block also applies the boundary terms of the mode
+// right-hand side (pot3.f:212-224) to the interior cells next to its point:
+// a top point J=j+2 <= JMAX takes CNMAX times its own coefficient off
+// (J, KMAX); an outer point K=k+2 < KMAX takes CMMAX times its own off
+// (JMAX, K); the top point J=JMAX also evaluates the outer point K=KMAX and
+// takes both terms off the corner (JMAX, KMAX), top first.  Every interior
+// cell has one writer.
+__global__ __launch_bounds__(64) void k_bdy_spec(BdyArgs a, double *spec, const double2 *twh, double *coef,
+                                                 int coef_mode, int edges)
+{
+    __shared__ double t1[kMaxTerms], t2[kMaxTerms];
+    __shared__ double cmv[kMaxTrm + 1], smv[kMaxTrm + 1];
+    __shared__ double cmv2[kMaxTrm + 1], smv2[kMaxTrm + 1];
+    const int jk = blockIdx.x;
+    const int tid = threadIdx.x;
+    point_sums(a, jk, t1, t2, cmv, smv);
     const int N = a.L / 2;
     const int j = a.jpos[jk] - 2, k = a.kpos[jk] - 2;
+    const int NZ = a.K - 1, M = a.J - 1;
+    const bool top = k == NZ;
+    const bool corner = edges && top && j == M - 1;
+    if (corner) point_sums(a, a.jkmax - 1, t1, t2, cmv2, smv2);  // outer point K=KMAX
     const size_t JK = (size_t)a.J * a.K, sidx = (size_t)k * a.J + j;
     const double g2 = -2.0 * a.grav;
     const double a0 = g2 * cmv[0];
     for (int q = tid; q < a.L; q += blockDim.x) {
-        double v = 0.0;
-        if (q == 0) {
-            v = a0;
-        } else if (q <= kMaxTrm) {
-            v = g2 * (cmv[q] * twh[q].x + smv[q] * twh[q].y);
-        } else if (q > N && q - N <= kMaxTrm) {
-            const int m = q - N;
-            v = g2 * (-cmv[m] * twh[m].y + smv[m] * twh[m].x);
-        }
+        const double v = packed_coef(cmv, smv, twh, q, N, g2);
         spec[(size_t)q * JK + sidx] = v;
+        if (edges) {
+            if (top && j <= M - 1) {
+                double *y = spec + (size_t)q * JK + (size_t)(NZ - 1) * a.J + j;
+                double yv = *y;
+                yv -= a.cnmax * v;
+                if (corner) yv -= a.cmmax * packed_coef(cmv2, smv2, twh, q, N, g2);
+                *y = yv;
+            } else if (!top && k < NZ - 1) {
+                spec[(size_t)q * JK + (size_t)k * a.J + (M - 1)] -= a.cmmax * v;
+            }
+        }
     }
     if (coef && (coef_mode == CHY_COEF_FULL || (coef_mode == CHY_COEF_K2 && k == 0))) {
         // raw REALTR coefficients are the packed ones times L (SF1 = 1/L)
@@ -335,11 +372,12 @@ int launch_bdygen_from_moments(const chy_grid *g, int maxtrm, int isym, double r
     a.J = g->J; a.K = g->K; a.L = g->L; a.JS = g->J + 2; a.KS = g->K + 2;
     a.nring = g->nring; a.nchunk = g->nchunk; a.jkmax = g->jkmax; a.nt = nt; a.isym = isym;
     a.redge = redge; a.grav = g->grav; a.pi = g->pi;
+    a.cnmax = g->cnmax; a.cmmax = g->cmmax;
     hipLaunchKernelGGL(k_bdy_chunks, dim3(g->nchunk), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_bdy_scan, dim3(2 * NCOL), dim3(256), 0, s, a);
     if (spectral) {
         hipLaunchKernelGGL(k_bdy_spec, dim3(g->jkmax), dim3(64), 0, s, a, g->d.spec, g->d.twh,
-                           coef_mode == CHY_COEF_NONE ? nullptr : coef, coef_mode);
+                           coef_mode == CHY_COEF_NONE ? nullptr : coef, coef_mode, 1);
     } else {
         hipLaunchKernelGGL(k_bdy_eval, dim3(g->jkmax), dim3(256), 0, s, a);
         hipLaunchKernelGGL(k_bdy_ghosts, dim3((g->L + 255) / 256), dim3(256), 0, s, a);

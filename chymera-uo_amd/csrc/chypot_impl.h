@@ -177,12 +177,15 @@ enum { LANES_ALL = 0, LANES_INTERIOR = 1, LANES_BOUNDARY = 2 };
 int launch_fft_forward(const chy_grid *g, const double *rho, const double *phi, double *coef,
                        int coef_mode, int lanes, bool moments, bool spectrum, hipStream_t s);
 int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s);
-int launch_mode_solve(const chy_grid *g, hipStream_t s, Profiler *prof);
+// Mode solves on d.spec; rhs_edges applies the boundary terms of Y first
+// (skipped when the spectral BDYGEN has applied them).
+int launch_mode_solve(const chy_grid *g, bool rhs_edges, hipStream_t s, Profiler *prof);
 // ZAXPHI with PHAC/PHICHK/KLOCAT (POT3's ghost cells come with the inverse FFT).
 int launch_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s);
 // BDYGEN from the ring moments in d.mom: radial split sums, then either the
 // boundary values and ghosts in PHI, or (spectral) the boundary lanes'
-// azimuthal spectrum written straight into d.spec with their COEF entries.
+// azimuthal spectrum written straight into d.spec with their COEF entries
+// and the boundary terms of the mode right-hand side.
 int launch_bdygen_from_moments(const chy_grid *g, int maxtrm, int isym, double redge, double *phi,
                                bool spectral, double *coef, int coef_mode, hipStream_t s);
 bool bdygen_spectral_ok(const chy_grid *g);

@@ -75,6 +75,19 @@ __device__ __forceinline__ void dft_reg(double2 (&v)[P], const double2 *__restri
     }
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by block index, and each
+// XCD has its own L2.  Neighbouring lane chunks share the cache lines at their
+// ends (rows of PHI are not 128-byte aligned), so chunk c goes to the block
+// that gives every XCD one contiguous run of chunks: those shared lines are
+// then fetched (or written back) once.  The grid is rounded up to a multiple
+// of 8; blocks past the last chunk leave at once.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_chunk(int nchunks)
+{
+    const int per = (nchunks + kXcds - 1) / kXcds;
+    return (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+}
+
 struct FftArgs {
     const double *rho;    // RHO (forward input on the interior)
     const double *phi;    // PHI (forward input on the boundary lanes)
@@ -122,7 +135,9 @@ __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
     const int tid = threadIdx.x;
     const int lane = tid % NL;
     const int sub = tid / NL;
-    const long lg = (long)blockIdx.x * NL + lane;
+    const int chunk = xcd_chunk((a.nlanes + NL - 1) / NL);
+    if (chunk * NL >= a.nlanes) return;  // whole block
+    const long lg = (long)chunk * NL + lane;
     const bool active = lg < a.nlanes;
     int j = 0, k = 0;
     lane_jk(a, active ? lg : 0, j, k);  // inactive lanes read lane 0's data and store nothing
@@ -244,7 +259,9 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
     const int tid = threadIdx.x;
     const int lane = tid % NL;
     const int sub = tid / NL;
-    const long lg = (long)blockIdx.x * NL + lane;
+    const int chunk = xcd_chunk((a.nlanes + NL - 1) / NL);
+    if (chunk * NL >= a.nlanes) return;  // whole block
+    const long lg = (long)chunk * NL + lane;
     const bool active = lg < a.nlanes;
     int j = 0, k = 0;
     lane_jk(a, active ? lg : 0, j, k);  // inactive lanes read lane 0's data and store nothing
@@ -349,7 +366,8 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
 template <int N1, int N2, int NL, bool FWD>
 int launch_one(const FftArgs &a, hipStream_t s)
 {
-    const int blocks = (a.nlanes + NL - 1) / NL;
+    const int chunks = (a.nlanes + NL - 1) / NL;
+    const int blocks = (chunks + kXcds - 1) / kXcds * kXcds;
     if (FWD) hipLaunchKernelGGL((k_fft_forward<N1, N2, NL>), dim3(blocks), dim3(NL * N2), 0, s, a);
     else hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL>), dim3(blocks), dim3(NL * N2), 0, s, a);
     return (int)hipGetLastError();

@@ -155,19 +155,18 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void k_zgemm(ModeArgs a)
 }
 
 // The same products for PP <= 128 (KMAX <= 128) with the weight table held in
-// registers.  One persistent block per CU walks output tiles of PPT rows x 64
-// columns (tile = (q, column block)); wave w owns rows 16w..16w+15, keeps its
-// 16 x PPT slice of the weights as MFMA A operands for the whole run, and
+// registers.  Persistent blocks (MINB per CU) walk output tiles of PPT rows x
+// PCT columns (tile = (q, column block)); wave w owns rows 16w..16w+15, keeps
+// its 16 x PPT slice of the weights as MFMA A operands for the whole run, and
 // takes B operands from a double-buffered LDS tile that the block fills with
 // the next tile's right-hand sides while the current one is multiplied.
-constexpr int PCT = 64;  // columns per tile
-
-template <bool FWD, int PPT>
-__global__ __launch_bounds__(PPT * 4) void k_zgemm_reg(ModeArgs a, int col_tiles, int ntiles)
+template <bool FWD, int PPT, int PCT, int MINB>
+__global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs a, int col_tiles, int ntiles)
 {
     constexpr int NT = PPT * 4;            // threads (PPT/16 waves)
     constexpr int KS = PPT / 4;            // k-steps of the 16x16x4 product
-    constexpr int BPER = PPT * PCT / NT;   // B elements fetched per thread (16)
+    constexpr int U = PCT / 16;            // 16-column MFMA tiles per wave
+    constexpr int BPER = PPT * PCT / NT;   // B elements fetched per thread
     __shared__ double Bs[2][PPT][PCT];
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6, lr = lane & 15, lk = lane >> 4;
@@ -194,22 +193,13 @@ __global__ __launch_bounds__(PPT * 4) void k_zgemm_reg(ModeArgs a, int col_tiles
     auto fetch = [&](int t) {
         const int q = t / col_tiles, j = (t % col_tiles) * PCT + cc;
         const int jc = j < a.M ? j : a.M - 1;
-        if (FWD) {
-            const double *S = a.spec + (size_t)q * a.J * a.K;
+        const double *src = FWD ? a.spec + (size_t)q * a.J * a.K : a.yh + (size_t)q * a.PP * a.JP;
+        const int pitch = FWD ? a.J : a.JP;
 #pragma unroll
-            for (int i = 0; i < BPER; ++i) {
-                const int kr = tid / PCT + (NT / PCT) * i;
-                const int kc = kr < nred ? kr : nred - 1;
-                rb[i] = S[(size_t)kc * a.J + jc];
-            }
-        } else {
-            const double *Z = a.yh + (size_t)q * a.PP * a.JP;
-#pragma unroll
-            for (int i = 0; i < BPER; ++i) {
-                const int kr = tid / PCT + (NT / PCT) * i;
-                const int kc = kr < nred ? kr : nred - 1;
-                rb[i] = Z[(size_t)kc * a.JP + jc];
-            }
+        for (int i = 0; i < BPER; ++i) {
+            const int kr = tid / PCT + (NT / PCT) * i;
+            const int kc = kr < nred ? kr : nred - 1;
+            rb[i] = src[(size_t)kc * pitch + jc];
         }
     };
     auto stash = [&](int buf, int t) {
@@ -228,20 +218,20 @@ __global__ __launch_bounds__(PPT * 4) void k_zgemm_reg(ModeArgs a, int col_tiles
     for (; tile < ntiles; tile += gridDim.x) {
         const int next = tile + gridDim.x;
         if (next < ntiles) fetch(next);
-        dx4 acc[4];
+        dx4 acc[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] = dx4{0.0, 0.0, 0.0, 0.0};
+        for (int u = 0; u < U; ++u) acc[u] = dx4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 const double b = Bs[buf][4 * ks + lk][16 * u + lr];
                 acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ks], b, acc[u], 0, 0, 0);
             }
         }
         const int q = tile / col_tiles, c0 = (tile % col_tiles) * PCT;
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int r = 16 * w + mrow(lane, v);
@@ -484,13 +474,20 @@ ModeArgs make_args(const chy_grid *g)
 template <bool FWD>
 void launch_zgemm(const chy_grid *g, const ModeArgs &a, hipStream_t s)
 {
-    const int col_tiles = ((FWD ? g->JP : g->M) + PCT - 1) / PCT;
-    const int ntiles = col_tiles * g->L;
-    const int blocks = std::min(ntiles, g->ncu);
-    if (g->PP == 128) {
-        hipLaunchKernelGGL((k_zgemm_reg<FWD, 128>), dim3(blocks), dim3(512), 0, s, a, col_tiles, ntiles);
-    } else if (g->PP == 64) {
-        hipLaunchKernelGGL((k_zgemm_reg<FWD, 64>), dim3(blocks), dim3(256), 0, s, a, col_tiles, ntiles);
+    // 64-column tiles, one persistent block per CU (128 KB / 64 KB of LDS)
+    constexpr int PCT128 = 64, PCT64 = 64;
+    if (g->PP == 128 || g->PP == 64) {
+        const int pct = g->PP == 128 ? PCT128 : PCT64;
+        const int minb = 1;
+        const int col_tiles = ((FWD ? g->JP : g->M) + pct - 1) / pct;
+        const int ntiles = col_tiles * g->L;
+        const int blocks = std::min(ntiles, g->ncu * minb);
+        if (g->PP == 128)
+            hipLaunchKernelGGL((k_zgemm_reg<FWD, 128, PCT128, 1>), dim3(blocks), dim3(512), 0, s, a, col_tiles,
+                               ntiles);
+        else
+            hipLaunchKernelGGL((k_zgemm_reg<FWD, 64, PCT64, 1>), dim3(blocks), dim3(256), 0, s, a, col_tiles,
+                               ntiles);
     } else if (FWD) {
         dim3 gf((g->JP + CT - 1) / CT, (g->PP + RT - 1) / RT, g->L);
         hipLaunchKernelGGL(k_zgemm<true>, gf, dim3(GEMM_THREADS), 0, s, a);
@@ -500,12 +497,12 @@ void launch_zgemm(const chy_grid *g, const ModeArgs &a, hipStream_t s)
     }
 }
 
-int launch_mode_solve(const chy_grid *g, hipStream_t s, Profiler *prof)
+int launch_mode_solve(const chy_grid *g, bool rhs_edges, hipStream_t s, Profiler *prof)
 {
     ModeArgs a = make_args(g);
     const int R = g->JP / 64;
     if (prof) prof->begin(CHY_STAGE_ZFWD, s);
-    {
+    if (rhs_edges) {
         const long n = (long)(g->M + g->NZ - 1) * g->L;
         hipLaunchKernelGGL(k_rhs_edges, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
     }
