@@ -131,6 +131,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=8, help="independent fields per step")
     ap.add_argument("--workers", type=int, default=0, help="worker streams (default min(batch, 8))")
+    ap.add_argument("--refine-steps", type=int, default=2, help="correction steps of the refined path")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the solves, copy a buffer of this many MiB device to device "
                          "three times (a kernel of known traffic for scaling HBM counters)")
@@ -205,6 +206,23 @@ def main():
     # single-solve latency: one field per call on the context's stream
     _, lat = timed(single, args.steps)
 
+    # the refined path (binary32 correction solves on a binary64 residual):
+    # single-field solves of field 0, timed the same way, and its error
+    # against the plain binary64 result of the timed region
+    ref_phi = torch.zeros_like(j0["phi"])
+    ref_coef = torch.zeros_like(j0["coef"])
+
+    def refined():
+        j0["grid"].solve_refined(j0["rho"], ref_phi, steps=args.refine_steps, coef=ref_coef,
+                                 coef_mode=chypot.COEF_K2)
+
+    for _ in range(args.warmup):
+        refined()
+    el_ref, lat_ref = timed(refined, args.steps)
+    _, rel_corr = j0["grid"].solve_refined(j0["rho"], ref_phi, steps=args.refine_steps, coef=ref_coef,
+                                           coef_mode=chypot.COEF_K2, report=True)
+    phi_ref = chypot.to_host(ref_phi)
+
     # single-field steps launched kernel by kernel with HIP events around every
     # stage on the solve's stream: per-kernel durations for the roofline
     ctx.profile(True)
@@ -221,6 +239,16 @@ def main():
         return float(np.abs(a - b).max() / scale)
 
     e_ref = max(err(phi[:, 1, :], fx["phi_k2"]), err(phi[:, :, 0], fx["phi_l1"]))
+    refined_line = {
+        "solves_per_s": args.steps / el_ref,
+        "p50_ms": float(np.percentile(lat_ref, 50)),
+        "p99_ms": float(np.percentile(lat_ref, 99)),
+        "steps": args.refine_steps,
+        "rel_correction_per_step": rel_corr,
+        "max_rel_err_vs_plain_fp64": float(np.abs(phi_ref - phi).max() / np.abs(phi).max()),
+        "max_rel_err_vs_reference_fp64": max(err(phi_ref[:, 1, :], fx["phi_k2"]),
+                                             err(phi_ref[:, :, 0], fx["phi_l1"])),
+    }
     e_ext = max(err(phi[:, 1, :], fx["ext_phi_k2"]), err(phi[:, :, 0], fx["ext_phi_l1"]))
 
     work = stage_work(g)
@@ -264,6 +292,7 @@ def main():
         "batch_step_p99_ms": float(np.percentile(lat_batch, 99)),
         "max_rel_err_vs_reference_fp64": e_ref,
         "max_rel_err_vs_exact": e_ext,
+        "refined_path": refined_line,
         "stages_ms": {k: round(v, 5) for k, v in stage_avg.items()},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -34,6 +34,7 @@ EXPORTS = (
     "chy_bdygen", "chy_pot3", "chy_solve", "chy_bdygen_host", "chy_pot3_host",
     "chy_ctx_profile", "chy_ctx_stage_times", "chy_ctx_graphs",
     "chy_zaxphi", "chy_zaxphi_host", "chy_phac", "chy_solve_batch", "chy_ctx_workers",
+    "chy_solve_refined",
     "chy_strerror", "chy_last_device_error",
 )
 
@@ -97,6 +98,8 @@ def lib():
         L.chy_phac.argtypes = [vp, dp, C.c_int]
         L.chy_solve_batch.argtypes = [vp, C.POINTER(SolveItem), C.c_int]
         L.chy_ctx_workers.argtypes = [vp, C.c_int]
+        L.chy_solve_refined.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp, C.c_int,
+                                        C.c_int, C.POINTER(Diag), dp]
         L.chy_ctx_stage_times.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int]
         L.chy_strerror.argtypes = [C.c_int]
         L.chy_strerror.restype = C.c_char_p
@@ -228,6 +231,17 @@ class PreparedGrid:
                                _dptr(rho), _dptr(phi), _dptr(coef), coef_mode,
                                C.byref(d) if d else None), "chy_solve")
         return d
+
+    def solve_refined(self, rho, phi, steps=2, maxtrm=10, isym=None, redge=0.0, npoint=8, coef=None,
+                      coef_mode=COEF_NONE, diag=None, report=False):
+        """BDYGEN + POT3 by refinement with binary32 correction solves.
+        Returns (Diag or None, per-step relative corrections or None)."""
+        d = diag if isinstance(diag, Diag) else (Diag() if diag else None)
+        rel = (C.c_double * steps)() if report else None
+        _check(lib().chy_solve_refined(self.handle, maxtrm, self.isym if isym is None else isym, redge, npoint,
+                                       _dptr(rho), _dptr(phi), _dptr(coef), coef_mode, steps,
+                                       C.byref(d) if d else None, rel), "chy_solve_refined")
+        return d, (list(rel) if report else None)
 
     def zaxphi(self, phi, npoint=8, klocat=0):
         """ZAXPHI alone on a device PHI (pot3.f:311-399); returns the diagnostics."""

@@ -259,6 +259,51 @@ static int run_pot3(chy_grid *g, int npoint, const double *rho, double *phi, dou
     return run_pot3_tail(g, s, npoint, phi, klocat_in, true);
 }
 
+// Buffers of the refined solve, allocated on first use.
+static int ensure_refine(chy_grid *g)
+{
+    auto get = [&](void **p, size_t bytes) -> int {
+        if (*p) return CHY_OK;
+        if (hipMalloc(p, bytes) != hipSuccess) return CHY_ENOMEM;
+        if (hipMemset(*p, 0, bytes) != hipSuccess) return CHY_EDEVICE;
+        g->allocations.push_back(*p);
+        g->device_bytes += bytes;
+        return CHY_OK;
+    };
+    int rc = get((void **)&g->d.res32, sizeof(float) * field_elems(g));
+    if (!rc) rc = get((void **)&g->d.spec32, sizeof(float) * (size_t)g->L * g->K * g->J);
+    if (!rc) rc = get((void **)&g->d.yh32, sizeof(float) * (size_t)g->L * g->PP * g->JP);
+    if (!rc) rc = get((void **)&g->d.rstat, sizeof(unsigned long long) * 2 * kMaxRefine);
+    return rc;
+}
+
+// BDYGEN in binary64 (boundary values into PHI), then `steps` corrections:
+// binary64 residual of the discrete operator, correction solved with binary32
+// spectra, added to PHI in binary64; POT3's ghosts with the last step, ZAXPHI.
+static int run_refined(chy_grid *g, hipStream_t s, int maxtrm, int isym, double redge, int npoint,
+                       const double *rho, double *phi, double *coef, int coef_mode, int steps, int klocat_in)
+{
+    int rc = ensure_refine(g);
+    if (rc) return rc;
+    rc = dev_rc(launch_fft_forward(g, rho, phi, coef, coef_mode, LANES_INTERIOR, true, false, s),
+                "moment transform launch");
+    if (!rc) rc = dev_rc(launch_bdygen_from_moments(g, maxtrm, isym, redge, phi, false, nullptr, CHY_COEF_NONE, s),
+                         "bdygen launch");
+    if (!rc && coef_mode != CHY_COEF_NONE)  // COEF of the boundary lanes
+        rc = dev_rc(launch_fft_forward(g, rho, phi, coef, coef_mode, LANES_BOUNDARY, false, false, s),
+                    "boundary COEF launch");
+    for (int it = 0; it < steps && !rc; ++it) {
+        rc = dev_rc(launch_residual(g, rho, phi, it == 0, it == 0, s), "residual launch");
+        if (!rc) rc = dev_rc(launch_fft_forward_res32(g, g->d.res32, g->d.spec32, s), "correction FFT launch");
+        if (!rc) rc = dev_rc(launch_mode_solve32(g, g->d.spec32, g->d.yh32, s), "correction mode solve launch");
+        if (!rc)
+            rc = dev_rc(launch_fft_inverse32(g, g->d.spec32, phi, it > 0, it == steps - 1, g->d.rstat + 2 * it, s),
+                        "correction inverse FFT launch");
+    }
+    if (!rc) rc = dev_rc(launch_zaxphi(g, npoint, klocat_in, phi, s), "ZAXPHI launch");
+    return rc;
+}
+
 static int read_diag(chy_grid *g, chy_diag *diag, bool tmass, bool zax)
 {
     double h[3];
@@ -617,6 +662,35 @@ int chy_pot3_host(chy_grid *g, int npoint, const double *rho_h, double *phi_h, d
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return dev_rc((int)e, "device to host copy");
     return diag ? read_diag(g, diag, false, true) : CHY_OK;
+}
+
+int chy_solve_refined(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho_d,
+                      double *phi_d, double *coef_d, int coef_mode, int steps, chy_diag *diag, double *rel_corr)
+{
+    if (!g) return CHY_ESTATE;
+    if (!rho_d || !phi_d || steps < 1 || steps > kMaxRefine) return CHY_EINVAL;
+    int rc = check_pot3_args(g, npoint, coef_mode, coef_d);
+    if (rc) return rc;
+    if (!bdygen_args_ok(maxtrm, isym, redge)) return CHY_EINVAL;
+    DeviceScope ds(g->ctx->device);
+    hipStream_t s = g->ctx->stream;
+    rc = run_refined(g, s, maxtrm, isym, redge, npoint, rho_d, phi_d, coef_d, coef_mode, steps,
+                     diag ? diag->klocat : 0);
+    if (rc == CHY_OK && rel_corr) {
+        unsigned long long h[2 * kMaxRefine];
+        hipError_t e = hipMemcpyAsync(h, g->d.rstat, sizeof(h), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return dev_rc((int)e, "refinement statistics readback");
+        rel_corr[0] = 1.0;  // the first step's correction is the whole solution
+        for (int it = 1; it < steps; ++it) {
+            double dm, pm;
+            std::memcpy(&dm, &h[2 * it], sizeof(double));
+            std::memcpy(&pm, &h[2 * it + 1], sizeof(double));
+            rel_corr[it] = pm > 0.0 ? dm / pm : 0.0;
+        }
+    }
+    if (rc == CHY_OK && diag) rc = read_diag(g, diag, true, true);
+    return rc;
 }
 
 int chy_zaxphi(chy_grid *g, int npoint, double *phi_d, chy_diag *diag)

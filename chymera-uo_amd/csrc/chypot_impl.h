@@ -30,6 +30,8 @@ constexpr int kTermM[kMaxTerms] = {0, 1, 0, 2, 1, 3, 0, 2, 4, 1, 3, 5, 0, 2, 4, 
 constexpr int kChunkRings = 64;
 // Captured solve graphs kept per grid (one per distinct argument set).
 constexpr int kGraphCache = 8;
+// Most refinement steps of chy_solve_refined.
+constexpr int kMaxRefine = 8;
 
 struct Term {
     int l, m;
@@ -54,6 +56,16 @@ struct DeviceGrid {
     double *am = nullptr;        // [JP]
     double *cm = nullptr;        // [JP]
     double *bm = nullptr;        // [(N+1)][JP]
+    // the same operator in physical space for refinement residuals:
+    // C(j) = 1/(RHF DTHETA)^2, AN (AN(1)=0), CN (CN(NZ)=0; CNMAX separate)
+    double *rc_c = nullptr;      // [M]
+    double *rz_an = nullptr;     // [NZ]
+    double *rz_cn = nullptr;     // [NZ]
+    // refinement buffers (allocated on first use)
+    float *res32 = nullptr;      // residual, PHI layout
+    float *spec32 = nullptr;     // [L][KMAX][JMAX]
+    float *yh32 = nullptr;       // [L][PP][JP]
+    unsigned long long *rstat = nullptr;  // [2 * kMaxRefine]: max |dPHI|, max |PHI| per step
     // ZAXPHI abscissae (pot3.f:342-348)
     double *zx = nullptr;        // [11]
     // BDYGEN
@@ -177,9 +189,25 @@ enum { LANES_ALL = 0, LANES_INTERIOR = 1, LANES_BOUNDARY = 2 };
 int launch_fft_forward(const chy_grid *g, const double *rho, const double *phi, double *coef,
                        int coef_mode, int lanes, bool moments, bool spectrum, hipStream_t s);
 int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s);
+// Refinement (binary32 storage): residual (PHI layout, interior cells) ->
+// spectrum of the interior lanes; spectrum -> PHI interior, set or added to,
+// with POT3's ghost cells when `ghosts`; stat (if not null) receives
+// max |correction| and max |PHI| as the bit patterns of non-negative doubles.
+int launch_fft_forward_res32(const chy_grid *g, const float *res, float *spec32, hipStream_t s);
+int launch_fft_inverse32(const chy_grid *g, const float *spec32, double *phi, bool accumulate, bool ghosts,
+                         unsigned long long *stat, hipStream_t s);
 // Mode solves on d.spec; rhs_edges applies the boundary terms of Y first
 // (skipped when the spectral BDYGEN has applied them).
 int launch_mode_solve(const chy_grid *g, bool rhs_edges, hipStream_t s, Profiler *prof);
+// The same on binary32 spectrum/eigen-coordinate buffers (refinement
+// corrections: zero boundary values, so no boundary terms).
+int launch_mode_solve32(const chy_grid *g, float *spec32, float *yh32, hipStream_t s);
+// FP64 residual 4 pi G RHO - A PHI of the 7-point operator the mode solves
+// invert, on the interior cells, rounded to binary32 into d.res32 (PHI
+// layout).  first: the interior of PHI is taken as 0 (only the boundary
+// values are read).  clear_stat: also zero d.rstat.
+int launch_residual(const chy_grid *g, const double *rho, const double *phi, bool first, bool clear_stat,
+                    hipStream_t s);
 // ZAXPHI with PHAC/PHICHK/KLOCAT (POT3's ghost cells come with the inverse FFT).
 int launch_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s);
 // BDYGEN from the ring moments in d.mom: radial split sums, then either the

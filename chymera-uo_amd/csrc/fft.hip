@@ -89,10 +89,10 @@ __device__ __forceinline__ int xcd_chunk(int nchunks)
 }
 
 struct FftArgs {
-    const double *rho;    // RHO (forward input on the interior)
+    const void *rho;      // RHO (forward input on the interior; TI elements)
     const double *phi;    // PHI (forward input on the boundary lanes)
     double *phi_out;      // PHI (inverse output)
-    double *spec;         // [L][K][J], or null (forward: no spectrum written)
+    void *spec;           // [L][K][J] of TS, or null (forward: no spectrum written)
     double *coef;         // COEF(JMAX+2,KMAX+2,LMAX/2,2) or null
     double *mom;          // ring moments [nring][21] of interior lanes, or null
     const double *dv;     // DV per ring (boundary.f:516)
@@ -105,6 +105,8 @@ struct FftArgs {
     int coef_mode;
     double sf1, pi;
     double ys;            // factor on the interior lanes' spectrum (4 pi G)
+    int ghosts;           // inverse: also write POT3's axis/midplane ghost cells
+    unsigned long long *stat;  // inverse with accumulation: max |dPHI|, max |PHI| (as bits), or null
 };
 
 // lane index -> (j,k), j = J-2 (0..JMAX-1), k = K-2 (0..KMAX-1)
@@ -125,7 +127,9 @@ __device__ __forceinline__ void lane_jk(const FftArgs &a, long lg, int &j, int &
     }
 }
 
-template <int N1, int N2, int NL>
+// TI: element type of the forward input on the interior lanes; TS: storage
+// type of the spectrum.  The transform itself is computed in binary64.
+template <int N1, int N2, int NL, typename TI, typename TS>
 __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
 {
     constexpr int N = N1 * N2;
@@ -142,7 +146,8 @@ __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
     int j = 0, k = 0;
     lane_jk(a, active ? lg : 0, j, k);  // inactive lanes read lane 0's data and store nothing
     // interior lanes J<=JMAX, K<=KMAX carry the density (pot3.f:71-81)
-    const double *src = (j <= a.J - 2 && k <= a.K - 2) ? a.rho : a.phi;
+    const bool interior = j <= a.J - 2 && k <= a.K - 2;
+    const TI *rho = static_cast<const TI *>(a.rho);
     const size_t plane = (size_t)a.JS * a.KS;
     const size_t base = (size_t)(j + 1) + (size_t)a.JS * (size_t)(k + 1);
 
@@ -151,7 +156,8 @@ __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
         const int n = N2 * n1 + sub;
-        v[n1] = make_double2(src[base + plane * (size_t)(2 * n)], src[base + plane * (size_t)(2 * n + 1)]);
+        const size_t i0 = base + plane * (size_t)(2 * n), i1 = i0 + plane;
+        v[n1] = interior ? make_double2((double)rho[i0], (double)rho[i1]) : make_double2(a.phi[i0], a.phi[i1]);
     }
     for (int e = tid; e < L; e += N2 * NL) tw[e] = a.twL[e];
     __syncthreads();
@@ -215,17 +221,20 @@ __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
             }
         }
     }
-    if (!a.spec) return;
     const bool want_coef = a.coef && (a.coef_mode == CHY_COEF_FULL || (a.coef_mode == CHY_COEF_K2 && k == 0));
+    if (!a.spec && !want_coef) return;
+    TS *spec = static_cast<TS *>(a.spec);
     // interior lanes feed the mode right-hand side Y = 4 pi G PHI~ (pot3.f:212)
-    const double ys = (j <= a.J - 2 && k <= a.K - 2) ? a.ys : 1.0;
+    const double ys = interior ? a.ys : 1.0;
     double A0 = 0.0;
     if (want_coef) A0 = 2.0 * xm(0).x;
     for (int m = sub; m <= N; m += N2) {
         const double2 X = xm(m);
         const double A = 2.0 * X.x, B = 2.0 * X.y;
-        a.spec[(size_t)m * JK + sidx] = A * a.sf1 * ys;
-        if (m >= 1 && m <= N - 1) a.spec[(size_t)(N + m) * JK + sidx] = B * a.sf1 * ys;
+        if (spec) {
+            spec[(size_t)m * JK + sidx] = (TS)(A * a.sf1 * ys);
+            if (m >= 1 && m <= N - 1) spec[(size_t)(N + m) * JK + sidx] = (TS)(B * a.sf1 * ys);
+        }
         if (want_coef && m >= 1) {
             // COEF(J,K,m,1) amplitude, COEF(J,K,m,2) phase (pot3.f:105-132)
             const double x1 = fabs(A0);
@@ -248,7 +257,10 @@ __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
     }
 }
 
-template <int N1, int N2, int NL>
+// TS: storage type of the spectrum.  ACC: add the result to PHI (the
+// correction of a refinement step; a.stat collects max |dPHI| and max |PHI|)
+// instead of writing it.
+template <int N1, int N2, int NL, typename TS, bool ACC>
 __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
 {
     constexpr int N = N1 * N2;
@@ -271,14 +283,15 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
     // phase 0: C_0 = a_0/2, C_N = a_N/2 (both real, packed in slot 0),
     //          C_m = (a_m + i b_m)/2, m = 1..N-1.  Every load is issued
     //          before any is used.
+    const TS *spec = static_cast<const TS *>(a.spec);
     double av[NM], bv[NM];
 #pragma unroll
     for (int t = 0; t < NM; ++t) {
         const int m = sub + N2 * t;
         const int ma = m <= N ? m : N;
         const int mb = (m >= 1 && m <= N - 1) ? N + m : 0;
-        av[t] = a.spec[(size_t)ma * JK + sidx];
-        bv[t] = a.spec[(size_t)mb * JK + sidx];
+        av[t] = (double)spec[(size_t)ma * JK + sidx];
+        bv[t] = (double)spec[(size_t)mb * JK + sidx];
     }
     for (int e = tid; e < L; e += N2 * NL) tw[e] = a.twL[e];
 #pragma unroll
@@ -335,58 +348,108 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
         for (int n2 = 0; n2 < N2; ++n2) u[r][n2] = lds[(k1 * N2 + n2) * NL + lane];
         dft_reg<N2, -1>(u[r], a.tw64);
     }
-    if (!active) return;
     const size_t plane = (size_t)a.JS * a.KS;
     const size_t base = (size_t)(j + 1) + (size_t)a.JS * (size_t)(k + 1);
     // POT3's ghost cells from the same values (pot3.f:287-296): the J=2 lanes
     // also write the axis ghost PHI(1,K,L)=PHI(2,K,L+-N), the K=2 lanes the
     // midplane row PHI(J,1,L)=PHI(J,2,L), the (2,2) lane both and PHI(1,1,L)
-    const bool axis = j == 0, mid = k == 0;
-    auto put = [&](int l, double v) {
-        a.phi_out[base + plane * (size_t)l] = v;
-        if (mid) a.phi_out[base - a.JS + plane * (size_t)l] = v;
-        if (axis) {
-            const size_t lg = (size_t)((l + N) % L);
-            a.phi_out[base - 1 + plane * lg] = v;
-            if (mid) a.phi_out[base - 1 - a.JS + plane * lg] = v;
-        }
-    };
+    const bool axis = a.ghosts && j == 0, mid = a.ghosts && k == 0;
+    double dmax = 0.0, pmax = 0.0;
+    if (active) {
+    if (ACC) {  // the old values first, all at once
+        double old[R2][N2][2];
+#pragma unroll
+        for (int r = 0; r < R2; ++r)
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const int n = sub + N2 * r + N1 * k2;
+                old[r][k2][0] = a.phi_out[base + plane * (size_t)(2 * n)];
+                old[r][k2][1] = a.phi_out[base + plane * (size_t)(2 * n + 1)];
+            }
+#pragma unroll
+        for (int r = 0; r < R2; ++r)
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const double dx = u[r][k2].x, dy = u[r][k2].y;
+                dmax = fmax(dmax, fmax(fabs(dx), fabs(dy)));
+                u[r][k2].x = old[r][k2][0] + dx;
+                u[r][k2].y = old[r][k2][1] + dy;
+                pmax = fmax(pmax, fmax(fabs(u[r][k2].x), fabs(u[r][k2].y)));
+            }
+    }
 #pragma unroll
     for (int r = 0; r < R2; ++r) {
         const int k1 = sub + N2 * r;
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) {
             const int n = k1 + N1 * k2;
-            put(2 * n, u[r][k2].x);
-            put(2 * n + 1, u[r][k2].y);
+            double *p0 = a.phi_out + base + plane * (size_t)(2 * n);
+            const double v0 = u[r][k2].x, v1 = u[r][k2].y;
+            p0[0] = v0;
+            p0[plane] = v1;
+            if (mid) {
+                p0[-a.JS] = v0;
+                p0[plane - a.JS] = v1;
+            }
+            if (axis) {
+                const size_t l0 = (size_t)((2 * n + N) % L), l1 = (size_t)((2 * n + 1 + N) % L);
+                a.phi_out[base - 1 + plane * l0] = v0;
+                a.phi_out[base - 1 + plane * l1] = v1;
+                if (mid) {
+                    a.phi_out[base - 1 - a.JS + plane * l0] = v0;
+                    a.phi_out[base - 1 - a.JS + plane * l1] = v1;
+                }
+            }
+        }
+    }
+    }  // active
+    if (ACC && a.stat) {
+        // wavefront maxima, then one atomic per wavefront (non-negative
+        // doubles order like their bit patterns)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+            pmax = fmax(pmax, __shfl_xor(pmax, off, 64));
+        }
+        if ((tid & 63) == 0) {
+            atomicMax(&a.stat[0], (unsigned long long)__double_as_longlong(dmax));
+            atomicMax(&a.stat[1], (unsigned long long)__double_as_longlong(pmax));
         }
     }
 }
 
-template <int N1, int N2, int NL, bool FWD>
+// transform variants: the FP64 solve, and the refinement's binary32-storage
+// correction (residual in, spectrum in binary32, PHI updated in place)
+enum { FWD64 = 0, FWD32 = 1, INV64 = 2, INV32_SET = 3, INV32_ADD = 4 };
+
+template <int N1, int N2, int NL, int KIND>
 int launch_one(const FftArgs &a, hipStream_t s)
 {
     const int chunks = (a.nlanes + NL - 1) / NL;
     const int blocks = (chunks + kXcds - 1) / kXcds * kXcds;
-    if (FWD) hipLaunchKernelGGL((k_fft_forward<N1, N2, NL>), dim3(blocks), dim3(NL * N2), 0, s, a);
-    else hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL>), dim3(blocks), dim3(NL * N2), 0, s, a);
+    const dim3 grid(blocks), block(NL * N2);
+    if (KIND == FWD64) hipLaunchKernelGGL((k_fft_forward<N1, N2, NL, double, double>), grid, block, 0, s, a);
+    if (KIND == FWD32) hipLaunchKernelGGL((k_fft_forward<N1, N2, NL, float, float>), grid, block, 0, s, a);
+    if (KIND == INV64) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, double, false>), grid, block, 0, s, a);
+    if (KIND == INV32_SET) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, float, false>), grid, block, 0, s, a);
+    if (KIND == INV32_ADD) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, float, true>), grid, block, 0, s, a);
     return (int)hipGetLastError();
 }
 
-template <bool FWD>
+template <int KIND>
 int dispatch(int N, const FftArgs &a, hipStream_t s)
 {
     switch (N) {
-    case 1: return launch_one<1, 1, 256, FWD>(a, s);
-    case 2: return launch_one<2, 1, 256, FWD>(a, s);
-    case 4: return launch_one<4, 1, 256, FWD>(a, s);
-    case 8: return launch_one<8, 1, 256, FWD>(a, s);
-    case 16: return launch_one<16, 1, 256, FWD>(a, s);
-    case 32: return launch_one<16, 2, 128, FWD>(a, s);
-    case 64: return launch_one<16, 4, 64, FWD>(a, s);
-    case 128: return launch_one<16, 8, 32, FWD>(a, s);
-    case 256: return launch_one<16, 16, 16, FWD>(a, s);
-    case 512: return launch_one<32, 16, 8, FWD>(a, s);
+    case 1: return launch_one<1, 1, 256, KIND>(a, s);
+    case 2: return launch_one<2, 1, 256, KIND>(a, s);
+    case 4: return launch_one<4, 1, 256, KIND>(a, s);
+    case 8: return launch_one<8, 1, 256, KIND>(a, s);
+    case 16: return launch_one<16, 1, 256, KIND>(a, s);
+    case 32: return launch_one<16, 2, 128, KIND>(a, s);
+    case 64: return launch_one<16, 4, 64, KIND>(a, s);
+    case 128: return launch_one<16, 8, 32, KIND>(a, s);
+    case 256: return launch_one<16, 16, 16, KIND>(a, s);
+    case 512: return launch_one<32, 16, 8, KIND>(a, s);
     default: return (int)hipErrorInvalidValue;
     }
 }
@@ -418,18 +481,39 @@ int launch_fft_forward(const chy_grid *g, const double *rho, const double *phi, 
     FftArgs a = make_args(g, lanes);
     a.rho = rho;
     a.phi = phi;
-    a.coef = (coef_mode == CHY_COEF_NONE || !spectrum) ? nullptr : coef;
+    a.coef = coef_mode == CHY_COEF_NONE ? nullptr : coef;
     a.coef_mode = coef_mode;
     a.mom = moments ? g->d.mom : nullptr;
     if (!spectrum) a.spec = nullptr;
-    return dispatch<true>(g->N, a, s);
+    return dispatch<FWD64>(g->N, a, s);
 }
 
 int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s)
 {
     FftArgs a = make_args(g, LANES_ALL);
     a.phi_out = phi;
-    return dispatch<false>(g->N, a, s);
+    a.ghosts = 1;
+    return dispatch<INV64>(g->N, a, s);
+}
+
+int launch_fft_forward_res32(const chy_grid *g, const float *res, float *spec32, hipStream_t s)
+{
+    FftArgs a = make_args(g, LANES_INTERIOR);
+    a.rho = res;
+    a.spec = spec32;
+    a.ys = 1.0;  // the residual already carries 4 pi G
+    return dispatch<FWD32>(g->N, a, s);
+}
+
+int launch_fft_inverse32(const chy_grid *g, const float *spec32, double *phi, bool accumulate, bool ghosts,
+                         unsigned long long *stat, hipStream_t s)
+{
+    FftArgs a = make_args(g, LANES_INTERIOR);
+    a.spec = const_cast<float *>(spec32);
+    a.phi_out = phi;
+    a.ghosts = ghosts ? 1 : 0;
+    a.stat = stat;
+    return accumulate ? dispatch<INV32_ADD>(g->N, a, s) : dispatch<INV32_SET>(g->N, a, s);
 }
 
 }  // namespace chy

@@ -29,8 +29,8 @@ namespace chy {
 namespace {
 
 struct ModeArgs {
-    double *spec;          // [L][K][J]
-    double *yh;            // [L][PP][JP]
+    void *spec;            // [L][K][J] of TS (binary64, or binary32 for refinement)
+    void *yh;              // [L][PP][JP] of TS
     const double *wf;      // [NZ][PP]  wf[k][p]
     const double *wi;      // [NZ][PP]  wi[p][k]
     const double *lam;     // [PP]
@@ -58,7 +58,7 @@ constexpr int B_PER = KT * CT / GEMM_THREADS;  // 8
 
 __device__ __forceinline__ int mrow(int lane, int v) { return (lane >> 4) + 4 * v; }
 
-template <bool FWD>
+template <bool FWD, typename TS>
 __global__ __launch_bounds__(GEMM_THREADS, 2) void k_zgemm(ModeArgs a)
 {
     __shared__ double As[KT][RT + 1];
@@ -69,8 +69,9 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void k_zgemm(ModeArgs a)
     const int nred = a.NZ;
     const int nrow = FWD ? a.PP : a.NZ;
     const double *W = FWD ? a.wf : a.wi;
-    const double *S = a.spec + (size_t)q * a.J * a.K;
-    const double *Z = a.yh + (size_t)q * a.PP * a.JP;
+    TS *const specp = static_cast<TS *>(a.spec), *const yhp = static_cast<TS *>(a.yh);
+    const TS *S = specp + (size_t)q * a.J * a.K;
+    const TS *Z = yhp + (size_t)q * a.PP * a.JP;
 
     double ra[A_PER], rb[B_PER];
     auto fetch = [&](int k0) {
@@ -147,9 +148,9 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void k_zgemm(ModeArgs a)
                 const int j = c0 + 16 * u + lr;
                 if (FWD) {
                     if (r < a.PP && j < a.JP)
-                        a.yh[((size_t)q * a.PP + r) * a.JP + j] = (r < a.NZ && j < a.M) ? acc[t][u][v] : 0.0;
+                        yhp[((size_t)q * a.PP + r) * a.JP + j] = (TS)((r < a.NZ && j < a.M) ? acc[t][u][v] : 0.0);
                 } else if (r < a.NZ && j < a.M) {
-                    a.spec[((size_t)q * a.K + r) * a.J + j] = acc[t][u][v];
+                    specp[((size_t)q * a.K + r) * a.J + j] = (TS)acc[t][u][v];
                 }
             }
 }
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void k_zgemm(ModeArgs a)
 // its 16 x PPT slice of the weights as MFMA A operands for the whole run, and
 // takes B operands from a double-buffered LDS tile that the block fills with
 // the next tile's right-hand sides while the current one is multiplied.
-template <bool FWD, int PPT, int PCT, int MINB>
+template <bool FWD, int PPT, int PCT, int MINB, typename TS>
 __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs a, int col_tiles, int ntiles)
 {
     constexpr int NT = PPT * 4;            // threads (PPT/16 waves)
@@ -193,13 +194,14 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
     auto fetch = [&](int t) {
         const int q = t / col_tiles, j = (t % col_tiles) * PCT + cc;
         const int jc = j < a.M ? j : a.M - 1;
-        const double *src = FWD ? a.spec + (size_t)q * a.J * a.K : a.yh + (size_t)q * a.PP * a.JP;
+        const TS *src = FWD ? static_cast<const TS *>(a.spec) + (size_t)q * a.J * a.K
+                            : static_cast<const TS *>(a.yh) + (size_t)q * a.PP * a.JP;
         const int pitch = FWD ? a.J : a.JP;
 #pragma unroll
         for (int i = 0; i < BPER; ++i) {
             const int kr = tid / PCT + (NT / PCT) * i;
             const int kc = kr < nred ? kr : nred - 1;
-            rb[i] = src[(size_t)kc * pitch + jc];
+            rb[i] = (double)src[(size_t)kc * pitch + jc];
         }
     };
     auto stash = [&](int buf, int t) {
@@ -238,9 +240,10 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
                 const int j = c0 + 16 * u + lr;
                 if (FWD) {
                     if (r < a.PP && j < a.JP)
-                        a.yh[((size_t)q * a.PP + r) * a.JP + j] = (r < a.NZ && j < a.M) ? acc[u][v] : 0.0;
+                        static_cast<TS *>(a.yh)[((size_t)q * a.PP + r) * a.JP + j] =
+                            (TS)((r < a.NZ && j < a.M) ? acc[u][v] : 0.0);
                 } else if (r < a.NZ && j < a.M) {
-                    a.spec[((size_t)q * a.K + r) * a.J + j] = acc[u][v];
+                    static_cast<TS *>(a.spec)[((size_t)q * a.K + r) * a.J + j] = (TS)acc[u][v];
                 }
             }
         if (next < ntiles) stash(buf ^ 1, next);
@@ -307,6 +310,37 @@ __device__ __forceinline__ void store_rows(double *p, const double (&d)[R])
     } else {
 #pragma unroll
         for (int i = 0; i < R; ++i) p[i] = d[i];
+    }
+}
+
+// The same for binary32 rows (refinement), widened to binary64.
+template <int R>
+__device__ __forceinline__ void load_rows(const float *p, double (&d)[R])
+{
+    if constexpr (R % 4 == 0) {
+        const float4 *p4 = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+        for (int i = 0; i < R / 4; ++i) {
+            const float4 v = p4[i];
+            d[4 * i] = v.x; d[4 * i + 1] = v.y; d[4 * i + 2] = v.z; d[4 * i + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) d[i] = p[i];
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void store_rows(float *p, const double (&d)[R])
+{
+    if constexpr (R % 4 == 0) {
+        float4 *p4 = reinterpret_cast<float4 *>(p);
+#pragma unroll
+        for (int i = 0; i < R / 4; ++i)
+            p4[i] = make_float4((float)d[4 * i], (float)d[4 * i + 1], (float)d[4 * i + 2], (float)d[4 * i + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) p[i] = (float)d[i];
     }
 }
 
@@ -403,7 +437,7 @@ __device__ __forceinline__ void rsolve_rows(const ModeArgs &a, int m, int p, int
 // Standalone r solves: one wavefront per (m, p) over the z-eigen coordinates
 // in yh, in place; the cosine (q=m) and sine (q=N+m, 1<=m<=N-1) right-hand
 // sides together.
-template <int R>
+template <int R, typename TS>
 __global__ __launch_bounds__(256) void k_rsolve(ModeArgs a)
 {
     const int lane = threadIdx.x & 63;
@@ -412,8 +446,8 @@ __global__ __launch_bounds__(256) void k_rsolve(ModeArgs a)
     if (sys >= (a.N + 1) * a.NZ) return;  // whole wavefront exits together
     const int m = sys / a.NZ, p = sys % a.NZ;
     const bool two = (m >= 1 && m <= a.N - 1);
-    double *y0 = a.yh + ((size_t)m * a.PP + p) * a.JP;
-    double *y1 = a.yh + ((size_t)(a.N + m) * a.PP + p) * a.JP;
+    TS *y0 = static_cast<TS *>(a.yh) + ((size_t)m * a.PP + p) * a.JP;
+    TS *y1 = static_cast<TS *>(a.yh) + ((size_t)(a.N + m) * a.PP + p) * a.JP;
     const int f = lane * R;
     // rows j >= M of yh are zero (written so by the z forward transform)
     double d0[R], d1[R], x0[R], x1[R];
@@ -439,7 +473,7 @@ __global__ __launch_bounds__(256) void k_rhs_edges(ModeArgs a)
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)ne * a.L) return;
     const int q = (int)(t / ne), e = (int)(t % ne);
-    double *S = a.spec + (size_t)q * a.J * a.K;
+    double *S = static_cast<double *>(a.spec) + (size_t)q * a.J * a.K;
     if (e < a.M) {
         const size_t c = (size_t)(a.NZ - 1) * a.J + e;
         double y = S[c];
@@ -469,60 +503,77 @@ ModeArgs make_args(const chy_grid *g)
     return a;
 }
 
-}  // namespace
-
-template <bool FWD>
+template <bool FWD, typename TS>
 void launch_zgemm(const chy_grid *g, const ModeArgs &a, hipStream_t s)
 {
     // 64-column tiles, one persistent block per CU (128 KB / 64 KB of LDS)
-    constexpr int PCT128 = 64, PCT64 = 64;
+    constexpr int PCT = 64;
     if (g->PP == 128 || g->PP == 64) {
-        const int pct = g->PP == 128 ? PCT128 : PCT64;
-        const int minb = 1;
-        const int col_tiles = ((FWD ? g->JP : g->M) + pct - 1) / pct;
+        const int col_tiles = ((FWD ? g->JP : g->M) + PCT - 1) / PCT;
         const int ntiles = col_tiles * g->L;
-        const int blocks = std::min(ntiles, g->ncu * minb);
+        const int blocks = std::min(ntiles, g->ncu);
         if (g->PP == 128)
-            hipLaunchKernelGGL((k_zgemm_reg<FWD, 128, PCT128, 1>), dim3(blocks), dim3(512), 0, s, a, col_tiles,
+            hipLaunchKernelGGL((k_zgemm_reg<FWD, 128, PCT, 1, TS>), dim3(blocks), dim3(512), 0, s, a, col_tiles,
                                ntiles);
         else
-            hipLaunchKernelGGL((k_zgemm_reg<FWD, 64, PCT64, 1>), dim3(blocks), dim3(256), 0, s, a, col_tiles,
+            hipLaunchKernelGGL((k_zgemm_reg<FWD, 64, PCT, 1, TS>), dim3(blocks), dim3(256), 0, s, a, col_tiles,
                                ntiles);
     } else if (FWD) {
         dim3 gf((g->JP + CT - 1) / CT, (g->PP + RT - 1) / RT, g->L);
-        hipLaunchKernelGGL(k_zgemm<true>, gf, dim3(GEMM_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_zgemm<true, TS>), gf, dim3(GEMM_THREADS), 0, s, a);
     } else {
         dim3 gi((g->M + CT - 1) / CT, (g->NZ + RT - 1) / RT, g->L);
-        hipLaunchKernelGGL(k_zgemm<false>, gi, dim3(GEMM_THREADS), 0, s, a);
+        hipLaunchKernelGGL((k_zgemm<false, TS>), gi, dim3(GEMM_THREADS), 0, s, a);
     }
 }
+
+template <typename TS>
+int launch_rsolve(const chy_grid *g, const ModeArgs &a, hipStream_t s)
+{
+    const int nsys = (g->N + 1) * g->NZ;
+    const dim3 grid((nsys + 3) / 4), block(256);
+    switch (g->JP / 64) {
+    case 1: hipLaunchKernelGGL((k_rsolve<1, TS>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_rsolve<2, TS>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_rsolve<4, TS>), grid, block, 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_rsolve<8, TS>), grid, block, 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_rsolve<16, TS>), grid, block, 0, s, a); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+}  // namespace
 
 int launch_mode_solve(const chy_grid *g, bool rhs_edges, hipStream_t s, Profiler *prof)
 {
     ModeArgs a = make_args(g);
-    const int R = g->JP / 64;
     if (prof) prof->begin(CHY_STAGE_ZFWD, s);
     if (rhs_edges) {
         const long n = (long)(g->M + g->NZ - 1) * g->L;
         hipLaunchKernelGGL(k_rhs_edges, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
     }
-    launch_zgemm<true>(g, a, s);
+    launch_zgemm<true, double>(g, a, s);
     if (prof) prof->end(s);
-    const int nsys = (g->N + 1) * g->NZ;
-    const int blocks = (nsys + 3) / 4;
     if (prof) prof->begin(CHY_STAGE_RSOLVE, s);
-    switch (R) {
-    case 1: hipLaunchKernelGGL(k_rsolve<1>, dim3(blocks), dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_rsolve<2>, dim3(blocks), dim3(256), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(k_rsolve<4>, dim3(blocks), dim3(256), 0, s, a); break;
-    case 8: hipLaunchKernelGGL(k_rsolve<8>, dim3(blocks), dim3(256), 0, s, a); break;
-    case 16: hipLaunchKernelGGL(k_rsolve<16>, dim3(blocks), dim3(256), 0, s, a); break;
-    default: return (int)hipErrorInvalidValue;
-    }
+    int rc = launch_rsolve<double>(g, a, s);
     if (prof) prof->end(s);
+    if (rc) return rc;
     if (prof) prof->begin(CHY_STAGE_ZINV, s);
-    launch_zgemm<false>(g, a, s);
+    launch_zgemm<false, double>(g, a, s);
     if (prof) prof->end(s);
+    return (int)hipGetLastError();
+}
+
+int launch_mode_solve32(const chy_grid *g, float *spec32, float *yh32, hipStream_t s)
+{
+    ModeArgs a = make_args(g);
+    a.spec = spec32;
+    a.yh = yh32;
+    launch_zgemm<true, float>(g, a, s);
+    int rc = launch_rsolve<float>(g, a, s);
+    if (rc) return rc;
+    launch_zgemm<false, float>(g, a, s);
     return (int)hipGetLastError();
 }
 

@@ -449,6 +449,10 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
         CHY_TRY(upload(g, &d.bm, bmp));
     }
     CHY_TRY(upload(g, &d.zx, zx));
+    // the 7-point operator of the same system for refinement residuals
+    CHY_TRY(upload(g, &d.rc_c, c));
+    CHY_TRY(upload(g, &d.rz_an, an));
+    CHY_TRY(upload(g, &d.rz_cn, cn));
     CHY_TRY(upload(g, &d.ring_of_pos, order));
     CHY_TRY(upload(g, &d.tin, tin));
     CHY_TRY(upload(g, &d.tout, tout));

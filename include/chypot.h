@@ -166,6 +166,21 @@ int chy_bdygen_host(chy_grid *g, int maxtrm, int isym, double redge, const doubl
 int chy_pot3_host(chy_grid *g, int npoint, const double *rho_h, double *phi_h, double *coef_h,
                   int coef_mode, chy_diag *diag);
 
+/* ---- refined solve (no reference counterpart) ----
+ * The same results as chy_solve (PHI, COEF, diagnostics) by iterative
+ * refinement: BDYGEN and COEF in binary64 as chy_solve computes them, then
+ * `steps` (1..8) correction steps -- the binary64 residual of the discrete
+ * 7-point operator the mode solves invert, a correction solved with binary32
+ * spectrum and eigen-coordinate storage (binary64 arithmetic in the kernels,
+ * half the memory traffic), added to PHI in binary64 -- then POT3's ghost
+ * cells and ZAXPHI.  The first step starts from 0 in the interior (whatever
+ * phi_d held there is not read).  rel_corr (host, may be NULL) receives, per
+ * step, the largest correction relative to the largest |PHI| after it (1 for
+ * the first step).  Waits when diag or rel_corr is non-NULL. */
+int chy_solve_refined(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho_d,
+                      double *phi_d, double *coef_d, int coef_mode, int steps, chy_diag *diag,
+                      double *rel_corr);
+
 /* ---- ZAXPHI alone (pot/pot3.f:311-399) ----
  * The NPOINT-point extrapolation of PHI to the z axis from PHI(J=2..,K,L) and
  * PHI(J=2..,K,L+N): writes PHI(JMAX+2,K,L) and PHI(JMAX+2,K,L+N) for
