@@ -84,6 +84,62 @@ def hbm_traffic(stage):
     return None
 
 
+def hetero_batch(ctx, n, reps=3):
+    """One chy_solve_batch over n independent fields of mixed grids (the grid
+    and torus of fields.random_field for seeds 1000.., generated on the GPU),
+    one prepared grid per shape; fields per second over `reps` batches."""
+    import torch
+
+    import chypot
+    from chypot import fields
+    prepared, jobs, points = {}, [], 0
+    for i in range(n):
+        g, rho_d = fields.random_field_device(1000 + i)
+        key = (g.jmax, g.kmax, g.lmax)
+        if key not in prepared:
+            prepared[key] = chypot.PreparedGrid(ctx, g)
+        jobs.append(dict(grid=prepared[key], rho=rho_d, phi=torch.zeros_like(rho_d),
+                         coef=torch.zeros((2, g.lmax // 2, g.kmax + 2, g.jmax + 2), dtype=torch.float64,
+                                          device="cuda"),
+                         coef_mode=chypot.COEF_K2))
+        points += g.jmax * g.kmax * g.lmax
+    ctx.solve_batch(jobs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.solve_batch(jobs)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    out = {"fields": n, "distinct_grids": len(prepared), "batch_ms": dt * 1e3,
+           "fields_per_s": n / dt, "grid_points_per_s": points / dt}
+    for pg in prepared.values():
+        pg.close()
+    return out
+
+
+def serve_run(ctx, args):
+    """The serving loop (chypot.serve) over Poisson arrivals: throughput and
+    per-request latency from arrival to completion."""
+    import torch
+
+    from chypot import serve
+    mix = [((128, 64, 128), 0.6), ((256, 128, 256), 0.3), ((512, 256, 512), 0.1)]
+    dens = {shape: serve.density_pool(shape, 2, seed=17, on_device=True) for shape, _ in mix}
+    torch.cuda.synchronize()
+    reqs = serve.poisson_requests(args.serve_rate, args.serve, mix, dens, seed=5)
+    srv = serve.Server(ctx, hbm_budget=int(args.serve_budget_gib * (1 << 30)), max_batch=16, workers=8)
+    srv.prepare([shape for shape, _ in mix])
+    t0 = time.perf_counter()
+    done = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    srv.close()
+    out = serve.latency_summary(done)
+    out.update({"offered_rate_per_s": args.serve_rate, "seconds": args.serve,
+                "completed_per_s": len(done) / wall,
+                "mix": "60% 128x64x128, 30% 256x128x256, 10% 512x256x512"})
+    return out
+
+
 def cpu_baseline(g, rho, nrep):
     """Reference BDYGEN+POT3 (flang -O3 real-8 build of pot/*.f, oracle/_ref) on
     this host, 1 thread; falls back to the oracle's C restatement."""
@@ -132,6 +188,14 @@ def main():
     ap.add_argument("--batch", type=int, default=8, help="independent fields per step")
     ap.add_argument("--workers", type=int, default=0, help="worker streams (default min(batch, 8))")
     ap.add_argument("--refine-steps", type=int, default=2, help="correction steps of the refined path")
+    ap.add_argument("--hetero", type=int, default=0,
+                    help="also time one batch of this many heterogeneous fields (J in 64..256, "
+                         "K in 32..128, L in 64..256; GPU-generated tori)")
+    ap.add_argument("--serve", type=float, default=0.0,
+                    help="also run the serving loop for this many seconds of Poisson arrivals "
+                         "(60%% 128x64x128, 30%% 256x128x256, 10%% 512x256x512)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="arrivals per second")
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="HBM budget of the server")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the solves, copy a buffer of this many MiB device to device "
                          "three times (a kernel of known traffic for scaling HBM counters)")
@@ -268,6 +332,8 @@ def main():
                 "launches": stages[dom][1]}
 
     roof["traffic"] = hbm_traffic(dom)
+    hetero = hetero_batch(ctx, args.hetero) if args.hetero > 0 else None
+    serving = serve_run(ctx, args) if args.serve > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(g, rho, args.cpu_reps)
     out = {
         "metric": METRIC,
@@ -293,6 +359,8 @@ def main():
         "max_rel_err_vs_reference_fp64": e_ref,
         "max_rel_err_vs_exact": e_ext,
         "refined_path": refined_line,
+        "hetero_batch": hetero,
+        "serving": serving,
         "stages_ms": {k: round(v, 5) for k, v in stage_avg.items()},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -133,3 +133,38 @@ def random_field(seed: int, grids=((64, 128, 256), (32, 64, 128), (64, 128, 256)
     lognorm = np.exp(0.05 * rng.standard_normal(size=g.shape))
     f = torus(g, rc, a, n) * lognorm
     return g, _place(g, f)
+
+
+def torus_device(g: Grid, rc: float, a: float, n: float, sigma: float, seed: int):
+    """The torus field of random_field generated on the GPU as a torch tensor
+    of shape (LMAX, KMAX+2, JMAX+2) (the Fortran layout): polytropic torus
+    times exp(sigma * N(0,1)) noise from a seeded torch generator, floored at
+    GRIDLIM times its maximum on the interior, zero outside it.  For large
+    synthetic workloads (the noise differs from random_field's numpy stream)."""
+    import torch
+    dev = "cuda"
+    r = torch.tensor(g.rhf(), dtype=torch.float64, device=dev)[None, :]
+    z = torch.tensor(g.zhf(), dtype=torch.float64, device=dev)[:, None]
+    base = torch.clamp(1.0 - ((r - rc) ** 2 + z ** 2) / (a * a), min=0.0) ** n
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    noise = torch.randn((g.lmax, g.kmax + 2, g.jmax + 2), generator=gen, dtype=torch.float64, device=dev)
+    f = base[None] * torch.exp(sigma * noise)
+    out = torch.zeros_like(f)
+    inner = f[:, 1:g.kmax + 1, 1:g.jmax + 1]
+    out[:, 1:g.kmax + 1, 1:g.jmax + 1] = torch.clamp(inner, min=GRIDLIM * float(inner.max()))
+    return out
+
+
+def random_field_device(seed: int, grids=((64, 128, 256), (32, 64, 128), (64, 128, 256))):
+    """random_field's grid and torus parameters (same seed, same draws), the
+    field generated on the GPU by torus_device.  Returns (grid, tensor)."""
+    rng = np.random.default_rng(seed)
+    j = int(rng.choice(grids[0]))
+    k = int(rng.choice(grids[1]))
+    l = int(rng.choice(grids[2]))
+    g = unit_grid(j, k, l)
+    rc = float(rng.uniform(0.3, 0.6))
+    a = float(rng.uniform(0.15, 0.35))
+    n = float(rng.choice([1.0, 1.5, 2.5]))
+    return g, torus_device(g, rc, a, n, 0.05, seed)

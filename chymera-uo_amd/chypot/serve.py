@@ -45,12 +45,14 @@ def field_bytes(shape) -> int:
 
 
 class Server:
-    def __init__(self, ctx: Context, hbm_budget: int, max_batch: int = 16, workers: int = 4):
+    def __init__(self, ctx: Context, hbm_budget: int, max_batch: int = 16, workers: int = 4,
+                 keep_outputs: bool = False):
         import torch
         self.torch = torch
         self.ctx = ctx
         self.budget = int(hbm_budget)
         self.max_batch = max_batch
+        self.keep_outputs = keep_outputs  # else a finished request's buffers are released
         ctx.workers(workers)
         # the context's stream: batches fork from and join back to it
         self.stream = torch.cuda.Stream()
@@ -112,6 +114,13 @@ class Server:
     def _release(self, req: Request):
         self.used -= self._request_bytes(req.shape)
         self.in_use[req.shape] -= 1
+        if not self.keep_outputs:
+            req.phi = req.coef = None
+
+    def prepare(self, shapes):
+        """Prepare grids ahead of traffic (SETBDY work kept off request latency)."""
+        for shape in shapes:
+            self._grid_for(tuple(shape))
 
     # -- loop -----------------------------------------------------------------
     def poll(self, now: float):
@@ -182,9 +191,10 @@ def poisson_requests(rate: float, duration: float, mix, densities, seed: int = 0
         used[shape] += 1
 
 
-def density_pool(shape, n: int, seed: int = 0):
+def density_pool(shape, n: int, seed: int = 0, on_device: bool = False):
     """n device-resident tori on a unit grid of `shape` (random centre,
-    radius, index and log-normal noise as in fields.random_field)."""
+    radius, index and log-normal noise as in fields.random_field; generated
+    on the GPU with on_device, for large grids)."""
     out = []
     for i in range(n):
         rng = np.random.default_rng(seed + 7919 * i)
@@ -192,6 +202,9 @@ def density_pool(shape, n: int, seed: int = 0):
         rc = float(rng.uniform(0.3, 0.6))
         a = float(rng.uniform(0.15, 0.35))
         nidx = float(rng.choice([1.0, 1.5, 2.5]))
+        if on_device:
+            out.append(fields.torus_device(g, rc, a, nidx, 0.05, seed + 7919 * i))
+            continue
         lognorm = np.exp(0.05 * rng.standard_normal(size=g.shape))
         rho = fields._place(g, fields.torus(g, rc, a, nidx) * lognorm)
         out.append(to_device(rho))

@@ -30,7 +30,7 @@ def test_alone_equals_mixed(ctx):
     reqs = serve.poisson_requests(rate=2000.0, duration=0.05, mix=MIX, densities=dens, seed=3)
     assert len(reqs) > 40
     print(f"[gpu] serving {len(reqs)} requests", flush=True)
-    srv = serve.Server(ctx, hbm_budget=8 << 30, max_batch=8, workers=4)
+    srv = serve.Server(ctx, hbm_budget=8 << 30, max_batch=8, workers=4, keep_outputs=True)
     done = srv.run(reqs)
     torch.cuda.synchronize()
     assert len(done) == len(reqs)
