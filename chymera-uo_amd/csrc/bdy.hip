@@ -72,11 +72,27 @@ __global__ __launch_bounds__(256) void k_bdy_chunks(BdyArgs a)
         to[it] = a.tout[(size_t)pc * kMaxTerms + t];
         rs[it] = a.rsph[pc];
     }
-    // ... while the block gathers its rings' moments (ring order in memory,
-    // radius order here) into LDS
-    for (int v = tid; v < (p1 - p0) * kMoments; v += blockDim.x) {
-        const int r = v / kMoments, cc = v % kMoments;
-        smom[r][cc] = a.mom[(size_t)a.ring_of_pos[p0 + r] * kMoments + cc];
+    // ... and the moments of the block's rings (ring order in memory, radius
+    // order in LDS): positions, then ring indices, then moments, each step's
+    // loads all issued before any is used
+    constexpr int GPER = (kChunkRings * kMoments + 255) / 256;
+    const int ne = (p1 - p0) * kMoments;
+    int gr[GPER];
+#pragma unroll
+    for (int i = 0; i < GPER; ++i) {
+        const int v = tid + 256 * i;
+        gr[i] = a.ring_of_pos[p0 + (v < ne ? v / kMoments : 0)];
+    }
+    double gm[GPER];
+#pragma unroll
+    for (int i = 0; i < GPER; ++i) {
+        const int v = tid + 256 * i;
+        gm[i] = a.mom[(size_t)gr[i] * kMoments + (v < ne ? v % kMoments : 0)];
+    }
+#pragma unroll
+    for (int i = 0; i < GPER; ++i) {
+        const int v = tid + 256 * i;
+        if (v < ne) smom[v / kMoments][v % kMoments] = gm[i];
     }
     __syncthreads();
     if (worker) {
@@ -114,34 +130,53 @@ __global__ __launch_bounds__(256) void k_bdy_chunks(BdyArgs a)
 // massio[c][0][col] = sum_{c'<c} chunk[c'][0][col]     (c = 0..nchunk)
 // massio[c][1][col] = sum_{c'>=c} chunk[c'][1][col]
 // One block per (column, component) reads its column (contiguous in the
-// [comp][col][chunk] layout) 256 chunks at a time: inclusive scans within
-// each wavefront by shuffles, then the wavefront totals in a fixed order.
+// [comp][col][chunk] layout) 1024 chunks at a time, every load of a pass
+// issued first: four consecutive chunks per thread summed in order, the
+// thread totals scanned within each wavefront by shuffles, then the wavefront
+// totals in a fixed order.
 __global__ __launch_bounds__(256) void k_bdy_scan(BdyArgs a)
 {
+    constexpr int E = 4;
     __shared__ double wsum[4];
     const int col = blockIdx.x % NCOL, comp = blockIdx.x / NCOL;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int n = a.nchunk;
     const double *src = a.chunk + ((size_t)comp * NCOL + col) * n;
     double carry = 0.0;
-    for (int base = 0; base < n; base += 256) {
-        const int i = base + tid;  // position in scan order
-        const int c = comp == 0 ? i : n - 1 - i;
-        const double x = src[i < n ? c : 0];
-        double v = i < n ? x : 0.0;
+    for (int base = 0; base < n; base += 256 * E) {
+        double v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = base + E * tid + e;  // position in scan order
+            const int c = comp == 0 ? i : n - 1 - i;
+            v[e] = src[i < n ? c : 0];
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (base + E * tid + e >= n) v[e] = 0.0;
+            if (e > 0) v[e] += v[e - 1];
+        }
+        double incl = v[E - 1];
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
-            const double u = __shfl_up(v, off, 64);
-            if (lane >= off) v += u;
+            const double u = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += u;
         }
-        if (lane == 63) wsum[w] = v;
+        double excl = __shfl_up(incl, 1, 64);
+        if (lane == 0) excl = 0.0;
+        if (lane == 63) wsum[w] = incl;
         __syncthreads();
         double pre = carry;
         for (int x2 = 0; x2 < w; ++x2) pre += wsum[x2];
-        const double incl = pre + v;
-        if (i < n) {
-            if (comp == 0) a.massio[((size_t)(i + 1) * 2 + 0) * NCOL + col] = incl;  // prefix of c' <= i
-            else a.massio[((size_t)(n - 1 - i) * 2 + 1) * NCOL + col] = incl;      // suffix from c = n-1-i
+        pre += excl;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = base + E * tid + e;
+            if (i < n) {
+                const double s = pre + v[e];
+                if (comp == 0) a.massio[((size_t)(i + 1) * 2 + 0) * NCOL + col] = s;  // prefix of c' <= i
+                else a.massio[((size_t)(n - 1 - i) * 2 + 1) * NCOL + col] = s;      // suffix from c = n-1-i
+            }
         }
         carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
         __syncthreads();
@@ -214,45 +249,39 @@ __global__ __launch_bounds__(256) void k_bdy_eval(BdyArgs a)
 //   b_m = -2 G (-C_m sin(m DTHETA/2) + S_m cos(m DTHETA/2))
 // and every other coefficient is 0.  Valid for LMAX/2 > MAXTRM.  Writes the
 // spectrum of the boundary lane and its COEF entries (pot3.f:105-132).
-// C_m, S_m (m = 0..10) of boundary point jk: the TRMTOT sums of
-// boundary.f:782-806 grouped by azimuthal order (block-cooperative, 64 threads)
-__device__ void point_sums(const BdyArgs &a, int jk, double *t1, double *t2, double *cmv, double *smv)
+// TRMTOT(1,t), TRMTOT(2,t) of boundary point jk for term t (boundary.f:782-806)
+__device__ __forceinline__ void point_terms(const BdyArgs &a, int jk, int t, double &v1, double &v2)
 {
-    const int tid = threadIdx.x;
-    if (tid < kMaxTerms) {
-        double v1 = 0.0, v2 = 0.0;
-        if (tid < a.nt) {
-            const double rs = a.rb[jk];
-            const int l = kTermL[tid];
-            double rout = 1.0;
-            for (int i = 0; i < l; ++i) rout *= rs;
-            const double rin = 1.0 / (rout * rs);
-            const double term = a.tb[(size_t)jk * kMaxTerms + tid];
-            const double b = term * rin, h = term * rout;
-            const int c = a.isym > 0 ? a.jk_cut[jk] : a.nchunk;
-            const double *mi = a.massio + ((size_t)c * 2 + 0) * NCOL;
-            const double *mo = a.massio + ((size_t)c * 2 + 1) * NCOL;
-            if (a.isym > 0) {
-                v1 = b * mi[2 * tid] + h * mo[2 * tid];
-                v2 = b * mi[2 * tid + 1] + h * mo[2 * tid + 1];
-            } else {
-                v1 = b * mi[2 * tid];
-                v2 = b * mi[2 * tid + 1];
-            }
-        }
-        t1[tid] = v1;
-        t2[tid] = v2;
+    v1 = 0.0;
+    v2 = 0.0;
+    if (t >= a.nt) return;
+    const double rs = a.rb[jk];
+    const double term = a.tb[(size_t)jk * kMaxTerms + t];
+    const int c = a.isym > 0 ? a.jk_cut[jk] : a.nchunk;
+    const int l = kTermL[t];
+    double rout = 1.0;
+    for (int i = 0; i < l; ++i) rout *= rs;
+    const double rin = 1.0 / (rout * rs);
+    const double b = term * rin, h = term * rout;
+    const double *mi = a.massio + ((size_t)c * 2 + 0) * NCOL;
+    const double *mo = a.massio + ((size_t)c * 2 + 1) * NCOL;
+    if (a.isym > 0) {
+        v1 = b * mi[2 * t] + h * mo[2 * t];
+        v2 = b * mi[2 * t + 1] + h * mo[2 * t + 1];
+    } else {  // all mass inside (boundary.f:795-805)
+        v1 = b * mi[2 * t];
+        v2 = b * mi[2 * t + 1];
     }
-    __syncthreads();
-    if (tid <= kMaxTrm) {
-        double c = 0.0, s = 0.0;
+}
+
+// C_m, S_m (m = 0..10): the TRMTOT sums grouped by azimuthal order
+__device__ __forceinline__ void order_sums(const double *t1, const double *t2, int m, double &c, double &s)
+{
+    c = 0.0;
+    s = 0.0;
 #pragma unroll
-        for (int t = 0; t < kMaxTerms; ++t)
-            if (kTermM[t] == tid) { c += t1[t]; s += t2[t]; }
-        cmv[tid] = c;
-        smv[tid] = s;
-    }
-    __syncthreads();
+    for (int t = 0; t < kMaxTerms; ++t)
+        if (kTermM[t] == m) { c += t1[t]; s += t2[t]; }
 }
 
 // packed coefficient q of the boundary series with sums cmv/smv
@@ -272,40 +301,64 @@ __device__ __forceinline__ double packed_coef(const double *cmv, const double *s
 // right-hand side (pot3.f:212-224) to the interior cells next to its point:
 // a top point J=j+2 <= JMAX takes CNMAX times its own coefficient off
 // (J, KMAX); an outer point K=k+2 < KMAX takes CMMAX times its own off
-// (JMAX, K); the top point J=JMAX also evaluates the outer point K=KMAX and
-// takes both terms off the corner (JMAX, KMAX), top first.  Every interior
-// cell has one writer.
-__global__ __launch_bounds__(64) void k_bdy_spec(BdyArgs a, double *spec, const double2 *twh, double *coef,
-                                                 int coef_mode, int edges)
+// (JMAX, K); the top point J=JMAX also evaluates the outer point K=KMAX
+// (second wavefront) and takes both terms off the corner (JMAX, KMAX), top
+// first.  Every interior cell has one writer.  The cells' old values are
+// loaded at the start, with the point's tables.
+constexpr int SPEC_QP = 4;  // packed coefficients per thread: LMAX <= 1024 over 256 threads
+__global__ __launch_bounds__(256) void k_bdy_spec(BdyArgs a, double *spec, const double2 *twh, double *coef,
+                                                  int coef_mode, int edges)
 {
-    __shared__ double t1[kMaxTerms], t2[kMaxTerms];
-    __shared__ double cmv[kMaxTrm + 1], smv[kMaxTrm + 1];
-    __shared__ double cmv2[kMaxTrm + 1], smv2[kMaxTrm + 1];
+    __shared__ double t1[2][kMaxTerms], t2[2][kMaxTerms];
+    __shared__ double cmv[2][kMaxTrm + 1], smv[2][kMaxTrm + 1];
     const int jk = blockIdx.x;
     const int tid = threadIdx.x;
-    point_sums(a, jk, t1, t2, cmv, smv);
     const int N = a.L / 2;
     const int j = a.jpos[jk] - 2, k = a.kpos[jk] - 2;
     const int NZ = a.K - 1, M = a.J - 1;
     const bool top = k == NZ;
     const bool corner = edges && top && j == M - 1;
-    if (corner) point_sums(a, a.jkmax - 1, t1, t2, cmv2, smv2);  // outer point K=KMAX
     const size_t JK = (size_t)a.J * a.K, sidx = (size_t)k * a.J + j;
+    // the interior cell that takes this point's edge term, if any
+    long ycell = -1;
+    if (edges) {
+        if (top && j <= M - 1) ycell = (long)(NZ - 1) * a.J + j;
+        else if (!top && k < NZ - 1) ycell = (long)k * a.J + (M - 1);
+    }
+    double yv[SPEC_QP];
+#pragma unroll
+    for (int i = 0; i < SPEC_QP; ++i) {
+        const int q = tid + 256 * i;
+        const size_t cell = ycell >= 0 ? (size_t)ycell : sidx;
+        yv[i] = spec[(size_t)(q < a.L ? q : 0) * JK + cell];
+    }
+    const int half = tid >> 6, tt = tid & 63;
+    if (tt < kMaxTerms && (half == 0 || (half == 1 && corner))) {
+        double v1, v2;
+        point_terms(a, half == 0 ? jk : a.jkmax - 1, tt, v1, v2);  // second: outer point K=KMAX
+        t1[half][tt] = v1;
+        t2[half][tt] = v2;
+    }
+    __syncthreads();
+    if (tt <= kMaxTrm && half < 2) order_sums(t1[half], t2[half], tt, cmv[half][tt], smv[half][tt]);
+    __syncthreads();
     const double g2 = -2.0 * a.grav;
-    const double a0 = g2 * cmv[0];
-    for (int q = tid; q < a.L; q += blockDim.x) {
-        const double v = packed_coef(cmv, smv, twh, q, N, g2);
+    const double a0 = g2 * cmv[0][0];
+#pragma unroll
+    for (int i = 0; i < SPEC_QP; ++i) {
+        const int q = tid + 256 * i;
+        if (q >= a.L) break;
+        const double v = packed_coef(cmv[0], smv[0], twh, q, N, g2);
         spec[(size_t)q * JK + sidx] = v;
-        if (edges) {
-            if (top && j <= M - 1) {
-                double *y = spec + (size_t)q * JK + (size_t)(NZ - 1) * a.J + j;
-                double yv = *y;
-                yv -= a.cnmax * v;
-                if (corner) yv -= a.cmmax * packed_coef(cmv2, smv2, twh, q, N, g2);
-                *y = yv;
-            } else if (!top && k < NZ - 1) {
-                spec[(size_t)q * JK + (size_t)k * a.J + (M - 1)] -= a.cmmax * v;
+        if (ycell >= 0) {
+            double y = yv[i];
+            if (top) {
+                y -= a.cnmax * v;
+                if (corner) y -= a.cmmax * packed_coef(cmv[1], smv[1], twh, q, N, g2);
+            } else {
+                y -= a.cmmax * v;
             }
+            spec[(size_t)q * JK + (size_t)ycell] = y;
         }
     }
     if (coef && (coef_mode == CHY_COEF_FULL || (coef_mode == CHY_COEF_K2 && k == 0))) {
@@ -316,8 +369,8 @@ __global__ __launch_bounds__(64) void k_bdy_spec(BdyArgs a, double *spec, const 
         for (int m = 1 + tid; m <= N; m += blockDim.x) {
             double A = 0.0, B = 0.0;
             if (m <= kMaxTrm && m < N) {
-                A = a.L * g2 * (cmv[m] * twh[m].x + smv[m] * twh[m].y);
-                B = a.L * g2 * (-cmv[m] * twh[m].y + smv[m] * twh[m].x);
+                A = a.L * g2 * (cmv[0][m] * twh[m].x + smv[0][m] * twh[m].y);
+                B = a.L * g2 * (-cmv[0][m] * twh[m].y + smv[0][m] * twh[m].x);
             }
             double *c1 = coef + base + plane * (size_t)(m - 1);
             double *c2 = c1 + plane * (size_t)N;
@@ -376,7 +429,7 @@ int launch_bdygen_from_moments(const chy_grid *g, int maxtrm, int isym, double r
     hipLaunchKernelGGL(k_bdy_chunks, dim3(g->nchunk), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_bdy_scan, dim3(2 * NCOL), dim3(256), 0, s, a);
     if (spectral) {
-        hipLaunchKernelGGL(k_bdy_spec, dim3(g->jkmax), dim3(64), 0, s, a, g->d.spec, g->d.twh,
+        hipLaunchKernelGGL(k_bdy_spec, dim3(g->jkmax), dim3(256), 0, s, a, g->d.spec, g->d.twh,
                            coef_mode == CHY_COEF_NONE ? nullptr : coef, coef_mode, 1);
     } else {
         hipLaunchKernelGGL(k_bdy_eval, dim3(g->jkmax), dim3(256), 0, s, a);
