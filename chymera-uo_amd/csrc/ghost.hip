@@ -10,6 +10,7 @@
 //             PHAC is 0 (pot3.f:391-399); it resets the counter for the next
 //             launch.
 #include "chypot_impl.h"
+#include "zax.h"
 
 namespace chy {
 namespace {
@@ -22,30 +23,6 @@ struct GhostArgs {
     int klocat_in;     // KLOCAT before the call (kept when every PHAC is 0)
     unsigned *ticket;  // blocks finished (0 between launches)
 };
-
-// (v, i) ordered by |v| first, then the smaller index: the reference's
-// sequential strict-greater scans keep the first of equal magnitudes.  A total
-// order, so any reduction tree gives the scan's answer.
-__device__ __forceinline__ void take_larger(double &v, int &i, double v2, int i2)
-{
-    if (fabs(v2) > fabs(v) || (fabs(v2) == fabs(v) && i2 < i)) { v = v2; i = i2; }
-}
-
-// argmax over the block (blockDim.x a multiple of 64, at most 256): wavefront
-// shuffles, then the wavefront winners through LDS.  Every thread gets the result.
-__device__ void block_argmax_abs(double v, int idx, double *sv, int *si, double &out_v, int &out_i)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        take_larger(v, idx, __shfl_xor(v, off, 64), __shfl_xor(idx, off, 64));
-    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if ((threadIdx.x & 63) == 0) { sv[w] = v; si[w] = idx; }
-    __syncthreads();
-    out_v = sv[0];
-    out_i = si[0];
-    for (int x = 1; x < nw; ++x) take_larger(out_v, out_i, sv[x], si[x]);
-    __syncthreads();  // sv/si free for the next call
-}
 
 // PPT: values of L (1..LMAX/2) per thread, blockDim.x * PPT >= LMAX/2
 template <int NP, int PPT>
@@ -60,97 +37,36 @@ __global__ __launch_bounds__(256) void k_zaxphi(GhostArgs a)
     const size_t row = (size_t)a.JS * (K - 1);
     double *phi = a.phi;
 
-    // ZAXPHI abscissae (pot3.f:336-348)
-    constexpr int np = NP;
-    constexpr int ichk = np % 2, nn = np - ichk, num = nn / 2, ispecl = num + num + 1, jsp = num + 1;
-    // every load first: the samples of this thread's L values and the abscissae
-    double p2[PPT][11];
+    // every load first: the samples of this thread's L values, then the abscissae
+    // PHI(J,K,L) and PHI(J,K,L+N) for J = 2..NUM+1, and PHI(NUM+2,K,L) for odd NP
+    constexpr int num = (NP - NP % 2) / 2;
+    double smp[PPT][2][num + 2];
 #pragma unroll
     for (int t = 0; t < PPT; ++t) {
         const int l = tid + (int)blockDim.x * t;
         const size_t lc = l < N ? (size_t)l : 0;
 #pragma unroll
-        for (int jj = 2; jj <= jsp; ++jj) {
-            p2[t][num + (jj - 1)] = phi[row + (jj - 1) + plane * lc];
-            p2[t][num - (jj - 2)] = phi[row + (jj - 1) + plane * (lc + N)];
+        for (int jj = 2; jj <= num + 1; ++jj) {
+            smp[t][0][jj - 1] = phi[row + (jj - 1) + plane * lc];
+            smp[t][1][jj - 1] = phi[row + (jj - 1) + plane * (lc + N)];
         }
-        if (ichk == 1) p2[t][ispecl] = phi[row + (num + 1) + plane * lc];
+        if (NP % 2 == 1) smp[t][0][num + 1] = phi[row + (num + 1) + plane * lc];
     }
     double x[11];
-#pragma unroll
-    for (int jj = 2; jj <= jsp; ++jj) {
-        x[num + (jj - 1)] = a.zx[jj - 1];
-        x[num - (jj - 2)] = -a.zx[jj - 1];
-    }
-    if (ichk == 1) x[ispecl] = a.zx[num + 1];
+    zax_abscissae<NP>(a.zx, x);
 
-    // extrapolated value PH(L) (pot3.f:354-379), same operations in the same order
     double ph[PPT];
-    double best = 0.0;
-    int besti = 0x7fffffff;
 #pragma unroll
     for (int t = 0; t < PPT; ++t) {
         const int l = tid + (int)blockDim.x * t;
-#pragma unroll
-        for (int i = 1; i <= np - 1; ++i) {
-            const double p1 = p2[t][i], xinv = 1.0 / x[i], xr = 0.0 * xinv;
-#pragma unroll
-            for (int jj = i + 1; jj <= np; ++jj) {
-                const double xratio = x[jj] * xinv, pb = p2[t][jj];
-                p2[t][jj] = (p1 * (xratio - xr) + pb * (xr - 1.0)) / (xratio - 1.0);
-            }
-        }
-        ph[t] = p2[t][np];
+        ph[t] = zax_extrapolate<NP>(x, [&](int jj, bool far) { return smp[t][far ? 1 : 0][jj - 1]; });
         if (l < N) {
             phi[row + (a.J + 1) + plane * l] = ph[t];
             phi[row + (a.J + 1) + plane * (l + N)] = ph[t];
-            take_larger(best, besti, ph[t], l);
         }
     }
-    double phmax;
-    int imax;
-    block_argmax_abs(best, besti, sv, si, phmax, imax);
-    // largest |1 - PH/PHMAX| (pot3.f:381-386); a zero deviation never sets
-    // the index, as the strict comparison against ERR=0 never takes one
-    double err = 0.0;
-    int erri = 0x7fffffff;
-#pragma unroll
-    for (int t = 0; t < PPT; ++t) {
-        const int l = tid + (int)blockDim.x * t;
-        const double er = 1.0 - ph[t] / phmax;
-        if (l < N && (fabs(er) > fabs(err) || (fabs(er) == fabs(err) && fabs(er) > 0.0 && l < erri))) {
-            err = er;
-            erri = l;
-        }
-    }
-    double e;
-    int ei;
-    block_argmax_abs(err, erri, sv, si, e, ei);
-
-    // PHICHK/KLOCAT by the last block to finish
-    __shared__ bool last;
-    if (tid == 0) {
-        a.dscal[4 + K] = e;
-        __threadfence();
-        last = atomicAdd(a.ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double best2 = 0.0;
-    int bk = 0x7fffffff;
-    for (int kk = 2 + tid; kk <= a.K + 1; kk += blockDim.x) {
-        const double v = __hip_atomic_load(&a.dscal[4 + kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (fabs(v) > fabs(best2)) { best2 = v; bk = kk; }
-    }
-    double pc;
-    int pk;
-    block_argmax_abs(best2, bk, sv, si, pc, pk);
-    if (tid == 0) {
-        a.dscal[1] = pc;
-        a.dscal[2] = fabs(pc) > 0.0 ? (double)pk : (double)a.klocat_in;
-        *a.ticket = 0u;
-    }
+    const double e = zax_phac<PPT>(ph, tid, (int)blockDim.x, N, sv, si);
+    zax_finish(a.dscal, K, e, a.ticket, gridDim.x, a.K, a.klocat_in, sv, si);
 }
 
 template <int NP>
