@@ -187,7 +187,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=8, help="independent fields per step")
     ap.add_argument("--workers", type=int, default=0, help="worker streams (default min(batch, 8))")
-    ap.add_argument("--refine-steps", type=int, default=2, help="correction steps of the refined path")
+    ap.add_argument("--refine-steps", type=int, default=2,
+                    help="correction steps of the refined path (0: do not measure it)")
     ap.add_argument("--hetero", type=int, default=0,
                     help="also time one batch of this many heterogeneous fields (J in 64..256, "
                          "K in 32..128, L in 64..256; GPU-generated tori)")
@@ -272,7 +273,8 @@ def main():
 
     # the refined path (binary32 correction solves on a binary64 residual):
     # single-field solves of field 0, timed the same way, and its error
-    # against the plain binary64 result of the timed region
+    # against the plain binary64 result of the timed region (skipped with
+    # --refine-steps 0, e.g. to profile the plain path alone)
     ref_phi = torch.zeros_like(j0["phi"])
     ref_coef = torch.zeros_like(j0["coef"])
 
@@ -280,12 +282,13 @@ def main():
         j0["grid"].solve_refined(j0["rho"], ref_phi, steps=args.refine_steps, coef=ref_coef,
                                  coef_mode=chypot.COEF_K2)
 
-    for _ in range(args.warmup):
-        refined()
-    el_ref, lat_ref = timed(refined, args.steps)
-    _, rel_corr = j0["grid"].solve_refined(j0["rho"], ref_phi, steps=args.refine_steps, coef=ref_coef,
-                                           coef_mode=chypot.COEF_K2, report=True)
-    phi_ref = chypot.to_host(ref_phi)
+    if args.refine_steps > 0:
+        for _ in range(args.warmup):
+            refined()
+        el_ref, lat_ref = timed(refined, args.steps)
+        _, rel_corr = j0["grid"].solve_refined(j0["rho"], ref_phi, steps=args.refine_steps, coef=ref_coef,
+                                               coef_mode=chypot.COEF_K2, report=True)
+        phi_ref = chypot.to_host(ref_phi)
 
     # single-field steps launched kernel by kernel with HIP events around every
     # stage on the solve's stream: per-kernel durations for the roofline
@@ -303,7 +306,7 @@ def main():
         return float(np.abs(a - b).max() / scale)
 
     e_ref = max(err(phi[:, 1, :], fx["phi_k2"]), err(phi[:, :, 0], fx["phi_l1"]))
-    refined_line = {
+    refined_line = None if args.refine_steps <= 0 else {
         "solves_per_s": args.steps / el_ref,
         "p50_ms": float(np.percentile(lat_ref, 50)),
         "p99_ms": float(np.percentile(lat_ref, 99)),
