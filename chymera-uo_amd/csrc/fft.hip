@@ -404,16 +404,28 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
     }
     }  // active
     if (ACC && a.stat) {
-        // wavefront maxima, then one atomic per wavefront (non-negative
-        // doubles order like their bit patterns)
+        // wavefront maxima, block maxima through LDS, then at most one atomic
+        // per block and statistic, skipped when the running maximum is already
+        // larger: every block raising the same two words one wavefront at a
+        // time serialises at the cache.  Non-negative doubles order like their
+        // bit patterns.
+        __shared__ double red[2][NL * N2 / 64];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
             pmax = fmax(pmax, __shfl_xor(pmax, off, 64));
         }
         if ((tid & 63) == 0) {
-            atomicMax(&a.stat[0], (unsigned long long)__double_as_longlong(dmax));
-            atomicMax(&a.stat[1], (unsigned long long)__double_as_longlong(pmax));
+            red[0][tid >> 6] = dmax;
+            red[1][tid >> 6] = pmax;
+        }
+        __syncthreads();
+        if (tid < 2) {
+            double m = red[tid][0];
+            for (int w = 1; w < NL * N2 / 64; ++w) m = fmax(m, red[tid][w]);
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+            unsigned long long *dst = &a.stat[tid];
+            if (bits > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(dst, bits);
         }
     }
 }
