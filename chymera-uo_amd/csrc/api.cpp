@@ -28,6 +28,38 @@ static int dev_rc(int hip_rc, const char *where)
     return CHY_EDEVICE;
 }
 
+void set_field(Fields &F, int f, const chy_grid *w, const void *rho, double *phi, double *coef, int klocat_in)
+{
+    F.rho[f] = rho;
+    F.phi[f] = phi;
+    F.coef[f] = coef;
+    F.spec[f] = w->d.spec;
+    F.yh[f] = w->d.yh;
+    F.mom[f] = w->d.mom;
+    F.chunk[f] = w->d.chunk;
+    F.massio[f] = w->d.massio;
+    F.dscal[f] = w->d.dscal;
+    F.ticket[f] = w->d.ticket;
+    F.klocat_in[f] = klocat_in;
+}
+
+Fields one_field(const chy_grid *g, const void *rho, double *phi, double *coef, int klocat_in)
+{
+    Fields F{};
+    F.nf = 1;
+    set_field(F, 0, g, rho, phi, coef, klocat_in);
+    return F;
+}
+
+// Grids prepared from identical descriptions have identical device tables, so
+// one launch can run fields of both (each on its own grid's workspace).
+static bool same_tables(const chy_grid *a, const chy_grid *b)
+{
+    return a->J == b->J && a->K == b->K && a->L == b->L && a->isym == b->isym && a->dtheta == b->dtheta &&
+           a->pi == b->pi && a->grav == b->grav && a->ncu == b->ncu && a->r == b->r && a->z == b->z &&
+           a->rhf == b->rhf && a->zhf == b->zhf;
+}
+
 static void free_grid_memory(chy_grid *g)
 {
     for (void *p : g->allocations) (void)hipFree(p);
@@ -246,6 +278,21 @@ static int run_solve(chy_grid *g, hipStream_t s, int maxtrm, int isym, double re
     return rc;
 }
 
+// BDYGEN + POT3 of every field in F (spectral BDYGEN), one launch per kernel
+// on grid g's tables: the sequence of run_solve_eager, field by field the
+// same arithmetic, so each field's result equals its own chy_solve's.
+static int run_solve_fields(chy_grid *g, const Fields &F, hipStream_t s, int maxtrm, int isym, double redge,
+                            int npoint, int coef_mode)
+{
+    int rc = dev_rc(launch_fft_forward_f(g, F, coef_mode, LANES_INTERIOR, true, true, s), "forward FFT launch");
+    if (!rc)
+        rc = dev_rc(launch_bdygen_from_moments_f(g, F, maxtrm, isym, redge, true, coef_mode, s), "bdygen launch");
+    if (!rc) rc = dev_rc(launch_mode_solve_f(g, F, false, s, nullptr), "mode solve launch");
+    if (!rc) rc = dev_rc(launch_fft_inverse_f(g, F, s), "inverse FFT launch");
+    if (!rc) rc = dev_rc(launch_zaxphi_f(g, F, npoint, s), "ZAXPHI launch");
+    return rc;
+}
+
 static int run_pot3(chy_grid *g, int npoint, const double *rho, double *phi, double *coef, int coef_mode,
                     int klocat_in)
 {
@@ -457,30 +504,76 @@ int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n)
         if (hipHostMalloc(&ctx->h_diag, sizeof(double) * 3 * n) != hipSuccess) return CHY_ENOMEM;
         ctx->diag_cap = n;
     }
-    // grids are dealt to the worker streams in order of first appearance;
-    // every solve on one grid runs on that grid's stream (they share its
-    // workspace), solves on different grids may overlap
-    std::vector<chy_grid *> seen;
-    std::vector<int> lane(n);
+    // Rounds: item i is in round r when r earlier items use its grid object
+    // (they share its workspace), so a round holds each grid object once and
+    // rounds run one after another.  Within a round, items on grids with the
+    // same tables and the same BDYGEN/POT3 parameters form groups of up to
+    // kMaxBatch fields that run as one launch per kernel; groups, and items
+    // alone, are dealt to the worker streams.
+    std::vector<int> round(n, 0);
+    int nrounds = 0;
     for (int i = 0; i < n; ++i) {
-        int k = 0;
-        while (k < (int)seen.size() && seen[k] != items[i].grid) ++k;
-        if (k == (int)seen.size()) seen.push_back(items[i].grid);
-        lane[i] = k % ctx->nworkers;
+        for (int j = 0; j < i; ++j)
+            if (items[j].grid == items[i].grid) ++round[i];
+        nrounds = std::max(nrounds, round[i] + 1);
     }
-    const int used = std::min((int)seen.size(), ctx->nworkers);
+    auto batchable = [](const chy_solve_item &a, const chy_solve_item &b) {
+        return same_tables(a.grid, b.grid) && bdygen_spectral_ok(a.grid) && a.maxtrm == b.maxtrm &&
+               a.isym == b.isym && a.redge == b.redge && std::min(a.npoint, 10) == std::min(b.npoint, 10) &&
+               a.coef_mode == b.coef_mode;
+    };
+    const int nw = ctx->nworkers;
+    int used = 0;
     rc = dev_rc((int)hipEventRecord(ctx->fork_ev, ctx->stream), "hipEventRecord");
-    for (int w = 0; w < used && rc == CHY_OK; ++w)
+    for (int w = 0; w < nw && rc == CHY_OK; ++w)
         rc = dev_rc((int)hipStreamWaitEvent(ctx->workers[w], ctx->fork_ev, 0), "hipStreamWaitEvent");
-    for (int i = 0; i < n && rc == CHY_OK; ++i) {
-        const chy_solve_item &it = items[i];
-        hipStream_t s = ctx->workers[lane[i]];
-        int np = std::min(it.npoint, 10);
-        rc = run_solve(it.grid, s, it.maxtrm, it.isym, it.redge, np, it.rho_d, it.phi_d, it.coef_d,
-                       it.coef_mode, it.diag ? it.diag->klocat : 0);
-        if (rc == CHY_OK && it.diag)
-            rc = dev_rc((int)hipMemcpyAsync(ctx->d_diag + 3 * i, it.grid->d.dscal, 3 * sizeof(double),
-                                            hipMemcpyDeviceToDevice, s), "diagnostics copy");
+    used = nw;
+    for (int r = 0; r < nrounds && rc == CHY_OK; ++r) {
+        if (r > 0) {
+            // every worker waits for every worker's previous round
+            for (int w = 0; w < nw && rc == CHY_OK; ++w)
+                rc = dev_rc((int)hipEventRecord(ctx->worker_ev[w], ctx->workers[w]), "hipEventRecord");
+            for (int w = 0; w < nw && rc == CHY_OK; ++w)
+                for (int v = 0; v < nw && rc == CHY_OK; ++v)
+                    if (v != w)
+                        rc = dev_rc((int)hipStreamWaitEvent(ctx->workers[w], ctx->worker_ev[v], 0),
+                                    "hipStreamWaitEvent");
+        }
+        std::vector<std::vector<int>> groups;
+        for (int i = 0; i < n; ++i) {
+            if (round[i] != r) continue;
+            size_t k = 0;
+            while (k < groups.size() &&
+                   !(groups[k].size() < (size_t)kMaxBatch && batchable(items[groups[k][0]], items[i])))
+                ++k;
+            if (k == groups.size()) groups.emplace_back();
+            groups[k].push_back(i);
+        }
+        for (size_t k = 0; k < groups.size() && rc == CHY_OK; ++k) {
+            hipStream_t s = ctx->workers[k % nw];
+            const std::vector<int> &grp = groups[k];
+            const chy_solve_item &lead = items[grp[0]];
+            const int np = std::min(lead.npoint, 10);
+            if (grp.size() == 1) {
+                rc = run_solve(lead.grid, s, lead.maxtrm, lead.isym, lead.redge, np, lead.rho_d, lead.phi_d,
+                               lead.coef_d, lead.coef_mode, lead.diag ? lead.diag->klocat : 0);
+            } else {
+                Fields F{};
+                F.nf = (int)grp.size();
+                for (int f = 0; f < F.nf; ++f) {
+                    const chy_solve_item &it = items[grp[f]];
+                    set_field(F, f, it.grid, it.rho_d, it.phi_d, it.coef_d, it.diag ? it.diag->klocat : 0);
+                }
+                rc = run_solve_fields(lead.grid, F, s, lead.maxtrm, lead.isym, lead.redge, np, lead.coef_mode);
+            }
+            for (size_t f = 0; f < grp.size() && rc == CHY_OK; ++f) {
+                const chy_solve_item &it = items[grp[f]];
+                if (it.diag)
+                    rc = dev_rc((int)hipMemcpyAsync(ctx->d_diag + 3 * grp[f], it.grid->d.dscal, 3 * sizeof(double),
+                                                    hipMemcpyDeviceToDevice, s),
+                                "diagnostics copy");
+            }
+        }
     }
     // join: the context's stream waits for every worker it forked
     for (int w = 0; w < used; ++w) {

@@ -49,11 +49,22 @@ struct BdyArgs {
     double cnmax, cmmax;          // boundary coefficients of the mode RHS (pot3.f:184-191)
 };
 
+// field f's arrays (blockIdx.y)
+__device__ __forceinline__ void bind(BdyArgs &a, const Fields &F, int f)
+{
+    a.phi = F.phi[f];
+    a.mom = F.mom[f];
+    a.chunk = F.chunk[f];
+    a.massio = F.massio[f];
+    a.dscal = F.dscal[f];
+}
+
 // chunk[0][2t+s][c]: sum over chunk c's rings of TERM_t * moment_s (inside
 // form), chunk[1][2t+s][c]: the same with TERM_t r^-(2l+1) (outside form);
 // s=0 cosine (SUMASS for m=0), s=1 sine.
-__global__ __launch_bounds__(256) void k_bdy_chunks(BdyArgs a)
+__global__ __launch_bounds__(256) void k_bdy_chunks(BdyArgs a, const Fields F)
 {
+    bind(a, F, blockIdx.y);
     constexpr int IT = (kChunkRings + CHUNK_GROUPS - 1) / CHUNK_GROUPS;  // rings per thread
     __shared__ double smom[kChunkRings][kMoments + 1];
     __shared__ double part[CHUNK_GROUPS][kMaxTerms][4];
@@ -134,8 +145,9 @@ __global__ __launch_bounds__(256) void k_bdy_chunks(BdyArgs a)
 // issued first: four consecutive chunks per thread summed in order, the
 // thread totals scanned within each wavefront by shuffles, then the wavefront
 // totals in a fixed order.
-__global__ __launch_bounds__(256) void k_bdy_scan(BdyArgs a)
+__global__ __launch_bounds__(256) void k_bdy_scan(BdyArgs a, const Fields F)
 {
+    bind(a, F, blockIdx.y);
     constexpr int E = 4;
     __shared__ double wsum[4];
     const int col = blockIdx.x % NCOL, comp = blockIdx.x / NCOL;
@@ -188,8 +200,9 @@ __global__ __launch_bounds__(256) void k_bdy_scan(BdyArgs a)
     }
 }
 
-__global__ __launch_bounds__(256) void k_bdy_eval(BdyArgs a)
+__global__ __launch_bounds__(256) void k_bdy_eval(BdyArgs a, const Fields F)
 {
+    bind(a, F, blockIdx.y);
     __shared__ double t1[kMaxTerms], t2[kMaxTerms];
     const int jk = blockIdx.x;
     const int tid = threadIdx.x;
@@ -306,9 +319,12 @@ __device__ __forceinline__ double packed_coef(const double *cmv, const double *s
 // first.  Every interior cell has one writer.  The cells' old values are
 // loaded at the start, with the point's tables.
 constexpr int SPEC_QP = 4;  // packed coefficients per thread: LMAX <= 1024 over 256 threads
-__global__ __launch_bounds__(256) void k_bdy_spec(BdyArgs a, double *spec, const double2 *twh, double *coef,
-                                                  int coef_mode, int edges)
+__global__ __launch_bounds__(256) void k_bdy_spec(BdyArgs a, const Fields F, const double2 *twh, int coef_mode,
+                                                  int edges)
 {
+    bind(a, F, blockIdx.y);
+    double *spec = static_cast<double *>(F.spec[blockIdx.y]);
+    double *coef = coef_mode == CHY_COEF_NONE ? nullptr : F.coef[blockIdx.y];
     __shared__ double t1[2][kMaxTerms], t2[2][kMaxTerms];
     __shared__ double cmv[2][kMaxTrm + 1], smv[2][kMaxTrm + 1];
     const int jk = blockIdx.x;
@@ -391,8 +407,9 @@ __global__ __launch_bounds__(256) void k_bdy_spec(BdyArgs a, double *spec, const
     }
 }
 
-__global__ __launch_bounds__(256) void k_bdy_ghosts(BdyArgs a)
+__global__ __launch_bounds__(256) void k_bdy_ghosts(BdyArgs a, const Fields F)
 {
+    bind(a, F, blockIdx.y);
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= a.L) return;
     const size_t plane = (size_t)a.JS * a.KS;
@@ -409,33 +426,39 @@ __global__ __launch_bounds__(256) void k_bdy_ghosts(BdyArgs a)
 
 bool bdygen_spectral_ok(const chy_grid *g) { return g->L / 2 > kMaxTrm; }
 
-int launch_bdygen_from_moments(const chy_grid *g, int maxtrm, int isym, double redge, double *phi,
-                               bool spectral, double *coef, int coef_mode, hipStream_t s)
+int launch_bdygen_from_moments_f(const chy_grid *g, const Fields &F, int maxtrm, int isym, double redge,
+                                 bool spectral, int coef_mode, hipStream_t s)
 {
     int nt = 0;
     while (nt < kMaxTerms && kTermL[nt] <= maxtrm) ++nt;
     BdyArgs a{};
-    a.phi = phi;
     a.cosm = g->d.cosm; a.sinm = g->d.sinm;
-    a.mom = g->d.mom; a.ring_of_pos = g->d.ring_of_pos;
+    a.ring_of_pos = g->d.ring_of_pos;
     a.tin = g->d.tin; a.tout = g->d.tout; a.rsph = g->d.rsph;
-    a.chunk_start = g->d.chunk_start; a.chunk = g->d.chunk; a.massio = g->d.massio;
+    a.chunk_start = g->d.chunk_start;
     a.jk_cut = g->d.jk_cut; a.tb = g->d.tb; a.rb = g->d.rb;
-    a.jpos = g->d.jpos; a.kpos = g->d.kpos; a.dscal = g->d.dscal;
+    a.jpos = g->d.jpos; a.kpos = g->d.kpos;
     a.J = g->J; a.K = g->K; a.L = g->L; a.JS = g->J + 2; a.KS = g->K + 2;
     a.nring = g->nring; a.nchunk = g->nchunk; a.jkmax = g->jkmax; a.nt = nt; a.isym = isym;
     a.redge = redge; a.grav = g->grav; a.pi = g->pi;
     a.cnmax = g->cnmax; a.cmmax = g->cmmax;
-    hipLaunchKernelGGL(k_bdy_chunks, dim3(g->nchunk), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_bdy_scan, dim3(2 * NCOL), dim3(256), 0, s, a);
+    const int nf = F.nf;
+    hipLaunchKernelGGL(k_bdy_chunks, dim3(g->nchunk, nf), dim3(256), 0, s, a, F);
+    hipLaunchKernelGGL(k_bdy_scan, dim3(2 * NCOL, nf), dim3(256), 0, s, a, F);
     if (spectral) {
-        hipLaunchKernelGGL(k_bdy_spec, dim3(g->jkmax), dim3(256), 0, s, a, g->d.spec, g->d.twh,
-                           coef_mode == CHY_COEF_NONE ? nullptr : coef, coef_mode, 1);
+        hipLaunchKernelGGL(k_bdy_spec, dim3(g->jkmax, nf), dim3(256), 0, s, a, F, g->d.twh, coef_mode, 1);
     } else {
-        hipLaunchKernelGGL(k_bdy_eval, dim3(g->jkmax), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_bdy_ghosts, dim3((g->L + 255) / 256), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_bdy_eval, dim3(g->jkmax, nf), dim3(256), 0, s, a, F);
+        hipLaunchKernelGGL(k_bdy_ghosts, dim3((g->L + 255) / 256, nf), dim3(256), 0, s, a, F);
     }
     return (int)hipGetLastError();
+}
+
+int launch_bdygen_from_moments(const chy_grid *g, int maxtrm, int isym, double redge, double *phi,
+                               bool spectral, double *coef, int coef_mode, hipStream_t s)
+{
+    const Fields F = one_field(g, nullptr, phi, coef, 0);
+    return launch_bdygen_from_moments_f(g, F, maxtrm, isym, redge, spectral, coef_mode, s);
 }
 
 }  // namespace chy

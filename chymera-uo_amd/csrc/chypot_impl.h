@@ -183,12 +183,41 @@ int symmetric_tridiag_eigen(int n, std::vector<long double> &d, std::vector<long
 // (BDYGEN's values).
 enum { LANES_ALL = 0, LANES_INTERIOR = 1, LANES_BOUNDARY = 2 };
 
-// kernel launchers (one .hip file each); all return hipError_t as int
+// Per-field pointers of a launch over nf fields that share one grid's tables
+// (identical grid descriptions): blockIdx.y, or the tile index of a
+// persistent kernel, picks field f.  Each field has the workspace of the grid
+// object it was prepared on, so fields on different grid objects run in one
+// launch without sharing scratch.  Single-field launches use nf = 1.
+constexpr int kMaxBatch = 16;
+struct Fields {
+    int nf;
+    const void *rho[kMaxBatch];  // forward input (binary64 density, or the binary32 residual)
+    double *phi[kMaxBatch];      // PHI: forward input on boundary lanes, inverse and BDYGEN output
+    double *coef[kMaxBatch];     // COEF (may be null)
+    void *spec[kMaxBatch];       // spectrum workspace (binary64, or binary32 for refinement)
+    void *yh[kMaxBatch];         // z-eigen workspace
+    double *mom[kMaxBatch];      // ring moments
+    double *chunk[kMaxBatch];    // BDYGEN chunk sums
+    double *massio[kMaxBatch];   // BDYGEN prefix/suffix sums
+    double *dscal[kMaxBatch];    // diagnostics
+    unsigned *ticket[kMaxBatch]; // ZAXPHI ticket counter
+    int klocat_in[kMaxBatch];    // KLOCAT before the call
+};
+// one field on grid g's own workspace
+Fields one_field(const chy_grid *g, const void *rho, double *phi, double *coef, int klocat_in);
+// field f of a batch: user arrays, and the workspace of grid w
+void set_field(Fields &F, int f, const chy_grid *w, const void *rho, double *phi, double *coef, int klocat_in);
+
+// kernel launchers (one .hip file each); all return hipError_t as int.  The
+// *_f forms run every field of F; the others one field on g's workspace.
 // Forward azimuthal transform; optionally writes BDYGEN's ring moments of the
 // interior lanes (d.mom) and/or the spectrum (d.spec, with COEF).
 int launch_fft_forward(const chy_grid *g, const double *rho, const double *phi, double *coef,
                        int coef_mode, int lanes, bool moments, bool spectrum, hipStream_t s);
 int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s);
+int launch_fft_forward_f(const chy_grid *g, const Fields &F, int coef_mode, int lanes, bool moments,
+                         bool spectrum, hipStream_t s);
+int launch_fft_inverse_f(const chy_grid *g, const Fields &F, hipStream_t s);
 // Refinement (binary32 storage): residual (PHI layout, interior cells) ->
 // spectrum of the interior lanes; spectrum -> PHI interior, set or added to,
 // with POT3's ghost cells when `ghosts`; stat (if not null) receives
@@ -199,6 +228,7 @@ int launch_fft_inverse32(const chy_grid *g, const float *spec32, double *phi, bo
 // Mode solves on d.spec; rhs_edges applies the boundary terms of Y first
 // (skipped when the spectral BDYGEN has applied them).
 int launch_mode_solve(const chy_grid *g, bool rhs_edges, hipStream_t s, Profiler *prof);
+int launch_mode_solve_f(const chy_grid *g, const Fields &F, bool rhs_edges, hipStream_t s, Profiler *prof);
 // The same on binary32 spectrum/eigen-coordinate buffers (refinement
 // corrections: zero boundary values, so no boundary terms).
 int launch_mode_solve32(const chy_grid *g, float *spec32, float *yh32, hipStream_t s);
@@ -210,12 +240,15 @@ int launch_residual(const chy_grid *g, const double *rho, const double *phi, boo
                     hipStream_t s);
 // ZAXPHI with PHAC/PHICHK/KLOCAT (POT3's ghost cells come with the inverse FFT).
 int launch_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s);
+int launch_zaxphi_f(const chy_grid *g, const Fields &F, int npoint, hipStream_t s);
 // BDYGEN from the ring moments in d.mom: radial split sums, then either the
 // boundary values and ghosts in PHI, or (spectral) the boundary lanes'
 // azimuthal spectrum written straight into d.spec with their COEF entries
 // and the boundary terms of the mode right-hand side.
 int launch_bdygen_from_moments(const chy_grid *g, int maxtrm, int isym, double redge, double *phi,
                                bool spectral, double *coef, int coef_mode, hipStream_t s);
+int launch_bdygen_from_moments_f(const chy_grid *g, const Fields &F, int maxtrm, int isym, double redge,
+                                 bool spectral, int coef_mode, hipStream_t s);
 bool bdygen_spectral_ok(const chy_grid *g);
 
 void set_device_error(hipError_t e, const char *where);

@@ -127,11 +127,23 @@ __device__ __forceinline__ void lane_jk(const FftArgs &a, long lg, int &j, int &
     }
 }
 
+// field f's arrays (blockIdx.y); a pointer left null by the launcher stays null
+__device__ __forceinline__ void bind(FftArgs &a, const Fields &F, int f)
+{
+    if (a.rho) a.rho = F.rho[f];
+    if (a.phi) a.phi = F.phi[f];
+    if (a.phi_out) a.phi_out = F.phi[f];
+    if (a.spec) a.spec = F.spec[f];
+    if (a.mom) a.mom = F.mom[f];
+    a.coef = a.coef_mode == CHY_COEF_NONE ? nullptr : F.coef[f];
+}
+
 // TI: element type of the forward input on the interior lanes; TS: storage
 // type of the spectrum.  The transform itself is computed in binary64.
 template <int N1, int N2, int NL, typename TI, typename TS>
-__global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
+__global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a, const Fields F)
 {
+    bind(a, F, blockIdx.y);
     constexpr int N = N1 * N2;
     constexpr int L = 2 * N;
     __shared__ double2 lds[N * NL];
@@ -261,8 +273,9 @@ __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a)
 // correction of a refinement step; a.stat collects max |dPHI| and max |PHI|)
 // instead of writing it.
 template <int N1, int N2, int NL, typename TS, bool ACC>
-__global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
+__global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a, const Fields F)
 {
+    bind(a, F, blockIdx.y);
     constexpr int N = N1 * N2;
     constexpr int L = 2 * N;
     constexpr int NM = N / N2 + 1;  // m = sub + N2 t, t < NM, covers 0..N
@@ -435,33 +448,33 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a)
 enum { FWD64 = 0, FWD32 = 1, INV64 = 2, INV32_SET = 3, INV32_ADD = 4 };
 
 template <int N1, int N2, int NL, int KIND>
-int launch_one(const FftArgs &a, hipStream_t s)
+int launch_one(const FftArgs &a, const Fields &F, hipStream_t s)
 {
     const int chunks = (a.nlanes + NL - 1) / NL;
     const int blocks = (chunks + kXcds - 1) / kXcds * kXcds;
-    const dim3 grid(blocks), block(NL * N2);
-    if (KIND == FWD64) hipLaunchKernelGGL((k_fft_forward<N1, N2, NL, double, double>), grid, block, 0, s, a);
-    if (KIND == FWD32) hipLaunchKernelGGL((k_fft_forward<N1, N2, NL, float, float>), grid, block, 0, s, a);
-    if (KIND == INV64) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, double, false>), grid, block, 0, s, a);
-    if (KIND == INV32_SET) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, float, false>), grid, block, 0, s, a);
-    if (KIND == INV32_ADD) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, float, true>), grid, block, 0, s, a);
+    const dim3 grid(blocks, F.nf), block(NL * N2);
+    if (KIND == FWD64) hipLaunchKernelGGL((k_fft_forward<N1, N2, NL, double, double>), grid, block, 0, s, a, F);
+    if (KIND == FWD32) hipLaunchKernelGGL((k_fft_forward<N1, N2, NL, float, float>), grid, block, 0, s, a, F);
+    if (KIND == INV64) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, double, false>), grid, block, 0, s, a, F);
+    if (KIND == INV32_SET) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, float, false>), grid, block, 0, s, a, F);
+    if (KIND == INV32_ADD) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, float, true>), grid, block, 0, s, a, F);
     return (int)hipGetLastError();
 }
 
 template <int KIND>
-int dispatch(int N, const FftArgs &a, hipStream_t s)
+int dispatch(int N, const FftArgs &a, const Fields &F, hipStream_t s)
 {
     switch (N) {
-    case 1: return launch_one<1, 1, 256, KIND>(a, s);
-    case 2: return launch_one<2, 1, 256, KIND>(a, s);
-    case 4: return launch_one<4, 1, 256, KIND>(a, s);
-    case 8: return launch_one<8, 1, 256, KIND>(a, s);
-    case 16: return launch_one<16, 1, 256, KIND>(a, s);
-    case 32: return launch_one<16, 2, 128, KIND>(a, s);
-    case 64: return launch_one<16, 4, 64, KIND>(a, s);
-    case 128: return launch_one<16, 8, 32, KIND>(a, s);
-    case 256: return launch_one<16, 16, 16, KIND>(a, s);
-    case 512: return launch_one<32, 16, 8, KIND>(a, s);
+    case 1: return launch_one<1, 1, 256, KIND>(a, F, s);
+    case 2: return launch_one<2, 1, 256, KIND>(a, F, s);
+    case 4: return launch_one<4, 1, 256, KIND>(a, F, s);
+    case 8: return launch_one<8, 1, 256, KIND>(a, F, s);
+    case 16: return launch_one<16, 1, 256, KIND>(a, F, s);
+    case 32: return launch_one<16, 2, 128, KIND>(a, F, s);
+    case 64: return launch_one<16, 4, 64, KIND>(a, F, s);
+    case 128: return launch_one<16, 8, 32, KIND>(a, F, s);
+    case 256: return launch_one<16, 16, 16, KIND>(a, F, s);
+    case 512: return launch_one<32, 16, 8, KIND>(a, F, s);
     default: return (int)hipErrorInvalidValue;
     }
 }
@@ -469,7 +482,6 @@ int dispatch(int N, const FftArgs &a, hipStream_t s)
 FftArgs make_args(const chy_grid *g, int lanes)
 {
     FftArgs a{};
-    a.spec = g->d.spec;
     a.twL = g->d.twL;
     a.tw64 = g->d.tw64;
     a.twh = g->d.twh;
@@ -487,45 +499,63 @@ FftArgs make_args(const chy_grid *g, int lanes)
 
 }  // namespace
 
+int launch_fft_forward_f(const chy_grid *g, const Fields &F, int coef_mode, int lanes, bool moments,
+                         bool spectrum, hipStream_t s)
+{
+    FftArgs a = make_args(g, lanes);
+    // non-null markers: bind() takes every field's own pointer
+    a.rho = F.rho[0];
+    a.phi = F.phi[0];
+    a.coef_mode = coef_mode;
+    a.mom = moments ? F.mom[0] : nullptr;
+    a.spec = spectrum ? F.spec[0] : nullptr;
+    return dispatch<FWD64>(g->N, a, F, s);
+}
+
 int launch_fft_forward(const chy_grid *g, const double *rho, const double *phi, double *coef,
                        int coef_mode, int lanes, bool moments, bool spectrum, hipStream_t s)
 {
-    FftArgs a = make_args(g, lanes);
-    a.rho = rho;
-    a.phi = phi;
-    a.coef = coef_mode == CHY_COEF_NONE ? nullptr : coef;
-    a.coef_mode = coef_mode;
-    a.mom = moments ? g->d.mom : nullptr;
-    if (!spectrum) a.spec = nullptr;
-    return dispatch<FWD64>(g->N, a, s);
+    const Fields F = one_field(g, rho, const_cast<double *>(phi), coef, 0);
+    return launch_fft_forward_f(g, F, coef_mode, lanes, moments, spectrum, s);
+}
+
+int launch_fft_inverse_f(const chy_grid *g, const Fields &F, hipStream_t s)
+{
+    FftArgs a = make_args(g, LANES_ALL);
+    a.spec = F.spec[0];
+    a.phi_out = F.phi[0];
+    a.ghosts = 1;
+    return dispatch<INV64>(g->N, a, F, s);
 }
 
 int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s)
 {
-    FftArgs a = make_args(g, LANES_ALL);
-    a.phi_out = phi;
-    a.ghosts = 1;
-    return dispatch<INV64>(g->N, a, s);
+    const Fields F = one_field(g, nullptr, phi, nullptr, 0);
+    return launch_fft_inverse_f(g, F, s);
 }
 
 int launch_fft_forward_res32(const chy_grid *g, const float *res, float *spec32, hipStream_t s)
 {
+    Fields F = one_field(g, res, nullptr, nullptr, 0);
+    F.spec[0] = spec32;
     FftArgs a = make_args(g, LANES_INTERIOR);
     a.rho = res;
     a.spec = spec32;
     a.ys = 1.0;  // the residual already carries 4 pi G
-    return dispatch<FWD32>(g->N, a, s);
+    return dispatch<FWD32>(g->N, a, F, s);
 }
 
 int launch_fft_inverse32(const chy_grid *g, const float *spec32, double *phi, bool accumulate, bool ghosts,
                          unsigned long long *stat, hipStream_t s)
 {
+    Fields F = one_field(g, nullptr, phi, nullptr, 0);
+    F.spec[0] = const_cast<float *>(spec32);
     FftArgs a = make_args(g, LANES_INTERIOR);
     a.spec = const_cast<float *>(spec32);
     a.phi_out = phi;
     a.ghosts = ghosts ? 1 : 0;
     a.stat = stat;
-    return accumulate ? dispatch<INV32_ADD>(g->N, a, s) : dispatch<INV32_SET>(g->N, a, s);
+    return accumulate ? dispatch<INV32_ADD>(g->N, a, F, s) : dispatch<INV32_SET>(g->N, a, F, s);
 }
 
 }  // namespace chy

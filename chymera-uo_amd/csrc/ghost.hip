@@ -26,8 +26,13 @@ struct GhostArgs {
 
 // PPT: values of L (1..LMAX/2) per thread, blockDim.x * PPT >= LMAX/2
 template <int NP, int PPT>
-__global__ __launch_bounds__(256) void k_zaxphi(GhostArgs a)
+__global__ __launch_bounds__(256) void k_zaxphi(GhostArgs a, const Fields F)
 {
+    // field f = blockIdx.y: its PHI, diagnostics and ticket
+    a.phi = F.phi[blockIdx.y];
+    a.dscal = F.dscal[blockIdx.y];
+    a.ticket = F.ticket[blockIdx.y];
+    a.klocat_in = F.klocat_in[blockIdx.y];
     __shared__ double sv[4];
     __shared__ int si[4];
     const int K = blockIdx.x + 2;  // Fortran K
@@ -70,43 +75,46 @@ __global__ __launch_bounds__(256) void k_zaxphi(GhostArgs a)
 }
 
 template <int NP>
-int launch_np(const chy_grid *g, const GhostArgs &a, hipStream_t s)
+int launch_np(const chy_grid *g, const GhostArgs &a, const Fields &F, hipStream_t s)
 {
     // one thread per L up to 256 threads (a whole number of wavefronts)
     const int N = g->L / 2;
     const int threads = N <= 64 ? 64 : (N <= 256 ? N : 256);
     const int ppt = (N + threads - 1) / threads;  // 1 or 2 (LMAX <= 1024)
-    if (ppt == 1) hipLaunchKernelGGL((k_zaxphi<NP, 1>), dim3(g->K), dim3(threads), 0, s, a);
-    else if (ppt == 2) hipLaunchKernelGGL((k_zaxphi<NP, 2>), dim3(g->K), dim3(threads), 0, s, a);
+    const dim3 grid(g->K, F.nf);
+    if (ppt == 1) hipLaunchKernelGGL((k_zaxphi<NP, 1>), grid, dim3(threads), 0, s, a, F);
+    else if (ppt == 2) hipLaunchKernelGGL((k_zaxphi<NP, 2>), grid, dim3(threads), 0, s, a, F);
     else return (int)hipErrorInvalidValue;
     return (int)hipGetLastError();
 }
 
 }  // namespace
 
-int launch_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s)
+int launch_zaxphi_f(const chy_grid *g, const Fields &F, int npoint, hipStream_t s)
 {
     GhostArgs a{};
-    a.klocat_in = klocat_in;
-    a.phi = phi;
     a.zx = g->d.zx;
-    a.dscal = g->d.dscal;
-    a.ticket = g->d.ticket;
     a.J = g->J; a.K = g->K; a.L = g->L; a.JS = g->J + 2; a.KS = g->K + 2;
     a.npoint = npoint;
     switch (npoint) {  // 1..10 (clamped by the callers)
-    case 1: return launch_np<1>(g, a, s);
-    case 2: return launch_np<2>(g, a, s);
-    case 3: return launch_np<3>(g, a, s);
-    case 4: return launch_np<4>(g, a, s);
-    case 5: return launch_np<5>(g, a, s);
-    case 6: return launch_np<6>(g, a, s);
-    case 7: return launch_np<7>(g, a, s);
-    case 8: return launch_np<8>(g, a, s);
-    case 9: return launch_np<9>(g, a, s);
-    case 10: return launch_np<10>(g, a, s);
+    case 1: return launch_np<1>(g, a, F, s);
+    case 2: return launch_np<2>(g, a, F, s);
+    case 3: return launch_np<3>(g, a, F, s);
+    case 4: return launch_np<4>(g, a, F, s);
+    case 5: return launch_np<5>(g, a, F, s);
+    case 6: return launch_np<6>(g, a, F, s);
+    case 7: return launch_np<7>(g, a, F, s);
+    case 8: return launch_np<8>(g, a, F, s);
+    case 9: return launch_np<9>(g, a, F, s);
+    case 10: return launch_np<10>(g, a, F, s);
     default: return (int)hipErrorInvalidValue;
     }
+}
+
+int launch_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s)
+{
+    const Fields F = one_field(g, nullptr, phi, nullptr, klocat_in);
+    return launch_zaxphi_f(g, F, npoint, s);
 }
 
 }  // namespace chy

@@ -59,11 +59,13 @@ constexpr int B_PER = KT * CT / GEMM_THREADS;  // 8
 __device__ __forceinline__ int mrow(int lane, int v) { return (lane >> 4) + 4 * v; }
 
 template <bool FWD, typename TS>
-__global__ __launch_bounds__(GEMM_THREADS, 2) void k_zgemm(ModeArgs a)
+__global__ __launch_bounds__(GEMM_THREADS, 2) void k_zgemm(ModeArgs a, const Fields F)
 {
     __shared__ double As[KT][RT + 1];
     __shared__ double Bs[KT][CT];
-    const int q = blockIdx.z;
+    const int q = blockIdx.z % a.L, f = blockIdx.z / a.L;  // mode, field
+    a.spec = F.spec[f];
+    a.yh = F.yh[f];
     const int r0 = blockIdx.y * RT, c0 = blockIdx.x * CT;
     const int tid = threadIdx.x;
     const int nred = a.NZ;
@@ -162,8 +164,11 @@ __global__ __launch_bounds__(GEMM_THREADS, 2) void k_zgemm(ModeArgs a)
 // takes B operands from a double-buffered LDS tile that the block fills with
 // the next tile's right-hand sides while the current one is multiplied.
 template <bool FWD, int PPT, int PCT, int MINB, typename TS>
-__global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs a, int col_tiles, int ntiles)
+__global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs a, const Fields F, int col_tiles,
+                                                                         int ntiles)
 {
+    // tile t: field t / tpf, then mode and column block within the field
+    const int tpf = col_tiles * a.L;
     constexpr int NT = PPT * 4;            // threads (PPT/16 waves)
     constexpr int KS = PPT / 4;            // k-steps of the 16x16x4 product
     constexpr int U = PCT / 16;            // 16-column MFMA tiles per wave
@@ -192,10 +197,11 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
     double rb[BPER];
     const int cc = tid % PCT;  // this thread's column within every tile
     auto fetch = [&](int t) {
-        const int q = t / col_tiles, j = (t % col_tiles) * PCT + cc;
+        const int f = t / tpf, tt = t % tpf;
+        const int q = tt / col_tiles, j = (tt % col_tiles) * PCT + cc;
         const int jc = j < a.M ? j : a.M - 1;
-        const TS *src = FWD ? static_cast<const TS *>(a.spec) + (size_t)q * a.J * a.K
-                            : static_cast<const TS *>(a.yh) + (size_t)q * a.PP * a.JP;
+        const TS *src = FWD ? static_cast<const TS *>(F.spec[f]) + (size_t)q * a.J * a.K
+                            : static_cast<const TS *>(F.yh[f]) + (size_t)q * a.PP * a.JP;
         const int pitch = FWD ? a.J : a.JP;
 #pragma unroll
         for (int i = 0; i < BPER; ++i) {
@@ -205,7 +211,7 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
         }
     };
     auto stash = [&](int buf, int t) {
-        const int j = (t % col_tiles) * PCT + cc;
+        const int j = ((t % tpf) % col_tiles) * PCT + cc;
 #pragma unroll
         for (int i = 0; i < BPER; ++i) {
             const int kr = tid / PCT + (NT / PCT) * i;
@@ -231,7 +237,10 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
                 acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[ks], b, acc[u], 0, 0, 0);
             }
         }
-        const int q = tile / col_tiles, c0 = (tile % col_tiles) * PCT;
+        const int f = tile / tpf, tt = tile % tpf;
+        const int q = tt / col_tiles, c0 = (tt % col_tiles) * PCT;
+        TS *const yhf = static_cast<TS *>(F.yh[f]);
+        TS *const specf = static_cast<TS *>(F.spec[f]);
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -240,10 +249,9 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
                 const int j = c0 + 16 * u + lr;
                 if (FWD) {
                     if (r < a.PP && j < a.JP)
-                        static_cast<TS *>(a.yh)[((size_t)q * a.PP + r) * a.JP + j] =
-                            (TS)((r < a.NZ && j < a.M) ? acc[u][v] : 0.0);
+                        yhf[((size_t)q * a.PP + r) * a.JP + j] = (TS)((r < a.NZ && j < a.M) ? acc[u][v] : 0.0);
                 } else if (r < a.NZ && j < a.M) {
-                    static_cast<TS *>(a.spec)[((size_t)q * a.K + r) * a.J + j] = (TS)acc[u][v];
+                    specf[((size_t)q * a.K + r) * a.J + j] = (TS)acc[u][v];
                 }
             }
         if (next < ntiles) stash(buf ^ 1, next);
@@ -438,8 +446,9 @@ __device__ __forceinline__ void rsolve_rows(const ModeArgs &a, int m, int p, int
 // in yh, in place; the cosine (q=m) and sine (q=N+m, 1<=m<=N-1) right-hand
 // sides together.
 template <int R, typename TS>
-__global__ __launch_bounds__(256) void k_rsolve(ModeArgs a)
+__global__ __launch_bounds__(256) void k_rsolve(ModeArgs a, const Fields F)
 {
+    a.yh = F.yh[blockIdx.y];  // field
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int sys = blockIdx.x * 4 + wave;
@@ -467,8 +476,9 @@ __global__ __launch_bounds__(256) void k_rsolve(ModeArgs a)
 // forward transform has already scaled by 4 pi G): the K=KMAX row loses
 // CNMAX times the top boundary lane, the J=JMAX column CMMAX times the outer
 // boundary lane.  One thread per (q, edge cell); the corner takes both.
-__global__ __launch_bounds__(256) void k_rhs_edges(ModeArgs a)
+__global__ __launch_bounds__(256) void k_rhs_edges(ModeArgs a, const Fields F)
 {
+    a.spec = F.spec[blockIdx.y];  // field
     const int ne = a.M + a.NZ - 1;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)ne * a.L) return;
@@ -489,8 +499,6 @@ __global__ __launch_bounds__(256) void k_rhs_edges(ModeArgs a)
 ModeArgs make_args(const chy_grid *g)
 {
     ModeArgs a{};
-    a.spec = g->d.spec;
-    a.yh = g->d.yh;
     a.wf = g->d.wf;
     a.wi = g->d.wi;
     a.lam = g->d.lam;
@@ -504,40 +512,40 @@ ModeArgs make_args(const chy_grid *g)
 }
 
 template <bool FWD, typename TS>
-void launch_zgemm(const chy_grid *g, const ModeArgs &a, hipStream_t s)
+void launch_zgemm(const chy_grid *g, const ModeArgs &a, const Fields &F, hipStream_t s)
 {
     // 64-column tiles, one persistent block per CU (128 KB / 64 KB of LDS)
     constexpr int PCT = 64;
     if (g->PP == 128 || g->PP == 64) {
         const int col_tiles = ((FWD ? g->JP : g->M) + PCT - 1) / PCT;
-        const int ntiles = col_tiles * g->L;
+        const int ntiles = col_tiles * g->L * F.nf;
         const int blocks = std::min(ntiles, g->ncu);
         if (g->PP == 128)
-            hipLaunchKernelGGL((k_zgemm_reg<FWD, 128, PCT, 1, TS>), dim3(blocks), dim3(512), 0, s, a, col_tiles,
+            hipLaunchKernelGGL((k_zgemm_reg<FWD, 128, PCT, 1, TS>), dim3(blocks), dim3(512), 0, s, a, F, col_tiles,
                                ntiles);
         else
-            hipLaunchKernelGGL((k_zgemm_reg<FWD, 64, PCT, 1, TS>), dim3(blocks), dim3(256), 0, s, a, col_tiles,
+            hipLaunchKernelGGL((k_zgemm_reg<FWD, 64, PCT, 1, TS>), dim3(blocks), dim3(256), 0, s, a, F, col_tiles,
                                ntiles);
     } else if (FWD) {
-        dim3 gf((g->JP + CT - 1) / CT, (g->PP + RT - 1) / RT, g->L);
-        hipLaunchKernelGGL((k_zgemm<true, TS>), gf, dim3(GEMM_THREADS), 0, s, a);
+        dim3 gf((g->JP + CT - 1) / CT, (g->PP + RT - 1) / RT, g->L * F.nf);
+        hipLaunchKernelGGL((k_zgemm<true, TS>), gf, dim3(GEMM_THREADS), 0, s, a, F);
     } else {
-        dim3 gi((g->M + CT - 1) / CT, (g->NZ + RT - 1) / RT, g->L);
-        hipLaunchKernelGGL((k_zgemm<false, TS>), gi, dim3(GEMM_THREADS), 0, s, a);
+        dim3 gi((g->M + CT - 1) / CT, (g->NZ + RT - 1) / RT, g->L * F.nf);
+        hipLaunchKernelGGL((k_zgemm<false, TS>), gi, dim3(GEMM_THREADS), 0, s, a, F);
     }
 }
 
 template <typename TS>
-int launch_rsolve(const chy_grid *g, const ModeArgs &a, hipStream_t s)
+int launch_rsolve(const chy_grid *g, const ModeArgs &a, const Fields &F, hipStream_t s)
 {
     const int nsys = (g->N + 1) * g->NZ;
-    const dim3 grid((nsys + 3) / 4), block(256);
+    const dim3 grid((nsys + 3) / 4, F.nf), block(256);
     switch (g->JP / 64) {
-    case 1: hipLaunchKernelGGL((k_rsolve<1, TS>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_rsolve<2, TS>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_rsolve<4, TS>), grid, block, 0, s, a); break;
-    case 8: hipLaunchKernelGGL((k_rsolve<8, TS>), grid, block, 0, s, a); break;
-    case 16: hipLaunchKernelGGL((k_rsolve<16, TS>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_rsolve<1, TS>), grid, block, 0, s, a, F); break;
+    case 2: hipLaunchKernelGGL((k_rsolve<2, TS>), grid, block, 0, s, a, F); break;
+    case 4: hipLaunchKernelGGL((k_rsolve<4, TS>), grid, block, 0, s, a, F); break;
+    case 8: hipLaunchKernelGGL((k_rsolve<8, TS>), grid, block, 0, s, a, F); break;
+    case 16: hipLaunchKernelGGL((k_rsolve<16, TS>), grid, block, 0, s, a, F); break;
     default: return (int)hipErrorInvalidValue;
     }
     return (int)hipGetLastError();
@@ -545,35 +553,42 @@ int launch_rsolve(const chy_grid *g, const ModeArgs &a, hipStream_t s)
 
 }  // namespace
 
-int launch_mode_solve(const chy_grid *g, bool rhs_edges, hipStream_t s, Profiler *prof)
+int launch_mode_solve_f(const chy_grid *g, const Fields &F, bool rhs_edges, hipStream_t s, Profiler *prof)
 {
     ModeArgs a = make_args(g);
     if (prof) prof->begin(CHY_STAGE_ZFWD, s);
     if (rhs_edges) {
         const long n = (long)(g->M + g->NZ - 1) * g->L;
-        hipLaunchKernelGGL(k_rhs_edges, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_rhs_edges, dim3((unsigned)((n + 255) / 256), F.nf), dim3(256), 0, s, a, F);
     }
-    launch_zgemm<true, double>(g, a, s);
+    launch_zgemm<true, double>(g, a, F, s);
     if (prof) prof->end(s);
     if (prof) prof->begin(CHY_STAGE_RSOLVE, s);
-    int rc = launch_rsolve<double>(g, a, s);
+    int rc = launch_rsolve<double>(g, a, F, s);
     if (prof) prof->end(s);
     if (rc) return rc;
     if (prof) prof->begin(CHY_STAGE_ZINV, s);
-    launch_zgemm<false, double>(g, a, s);
+    launch_zgemm<false, double>(g, a, F, s);
     if (prof) prof->end(s);
     return (int)hipGetLastError();
+}
+
+int launch_mode_solve(const chy_grid *g, bool rhs_edges, hipStream_t s, Profiler *prof)
+{
+    const Fields F = one_field(g, nullptr, nullptr, nullptr, 0);
+    return launch_mode_solve_f(g, F, rhs_edges, s, prof);
 }
 
 int launch_mode_solve32(const chy_grid *g, float *spec32, float *yh32, hipStream_t s)
 {
     ModeArgs a = make_args(g);
-    a.spec = spec32;
-    a.yh = yh32;
-    launch_zgemm<true, float>(g, a, s);
-    int rc = launch_rsolve<float>(g, a, s);
+    Fields F = one_field(g, nullptr, nullptr, nullptr, 0);
+    F.spec[0] = spec32;
+    F.yh[0] = yh32;
+    launch_zgemm<true, float>(g, a, F, s);
+    int rc = launch_rsolve<float>(g, a, F, s);
     if (rc) return rc;
-    launch_zgemm<false, float>(g, a, s);
+    launch_zgemm<false, float>(g, a, F, s);
     return (int)hipGetLastError();
 }
 
