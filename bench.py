@@ -185,7 +185,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batch", type=int, default=8, help="independent fields per step")
+    ap.add_argument("--batch", type=int, default=16, help="independent fields per step")
     ap.add_argument("--workers", type=int, default=0, help="worker streams (default min(batch, 8))")
     ap.add_argument("--refine-steps", type=int, default=2,
                     help="correction steps of the refined path (0: do not measure it)")

@@ -505,11 +505,12 @@ int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n)
         ctx->diag_cap = n;
     }
     // Rounds: item i is in round r when r earlier items use its grid object
-    // (they share its workspace), so a round holds each grid object once and
-    // rounds run one after another.  Within a round, items on grids with the
+    // (they share its workspace).  Within a round, items on grids with the
     // same tables and the same BDYGEN/POT3 parameters form groups of up to
-    // kMaxBatch fields that run as one launch per kernel; groups, and items
-    // alone, are dealt to the worker streams.
+    // kMaxBatch fields that run as one launch per kernel.  Every grid object
+    // has a worker stream (order of first appearance); a group runs on its
+    // first item's worker and waits only for the workers that last used its
+    // other grids, so work on unrelated grids never waits.
     std::vector<int> round(n, 0);
     int nrounds = 0;
     for (int i = 0; i < n; ++i) {
@@ -523,22 +524,23 @@ int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n)
                a.coef_mode == b.coef_mode;
     };
     const int nw = ctx->nworkers;
-    int used = 0;
-    rc = dev_rc((int)hipEventRecord(ctx->fork_ev, ctx->stream), "hipEventRecord");
-    for (int w = 0; w < nw && rc == CHY_OK; ++w)
-        rc = dev_rc((int)hipStreamWaitEvent(ctx->workers[w], ctx->fork_ev, 0), "hipStreamWaitEvent");
-    used = nw;
-    for (int r = 0; r < nrounds && rc == CHY_OK; ++r) {
-        if (r > 0) {
-            // every worker waits for every worker's previous round
-            for (int w = 0; w < nw && rc == CHY_OK; ++w)
-                rc = dev_rc((int)hipEventRecord(ctx->worker_ev[w], ctx->workers[w]), "hipEventRecord");
-            for (int w = 0; w < nw && rc == CHY_OK; ++w)
-                for (int v = 0; v < nw && rc == CHY_OK; ++v)
-                    if (v != w)
-                        rc = dev_rc((int)hipStreamWaitEvent(ctx->workers[w], ctx->worker_ev[v], 0),
-                                    "hipStreamWaitEvent");
+    std::vector<chy_grid *> seen;      // grid objects in order of first appearance (worker = index % nw)
+    std::vector<int> last_w;           // worker of each seen grid's latest use, -1 before it
+    auto grid_index = [&](chy_grid *g) {
+        size_t k = 0;
+        while (k < seen.size() && seen[k] != g) ++k;
+        if (k == seen.size()) {
+            seen.push_back(g);
+            last_w.push_back(-1);
         }
+        return (int)k;
+    };
+    for (int i = 0; i < n; ++i) (void)grid_index(items[i].grid);
+    const int used = std::min((int)seen.size(), nw);
+    rc = dev_rc((int)hipEventRecord(ctx->fork_ev, ctx->stream), "hipEventRecord");
+    for (int w = 0; w < used && rc == CHY_OK; ++w)
+        rc = dev_rc((int)hipStreamWaitEvent(ctx->workers[w], ctx->fork_ev, 0), "hipStreamWaitEvent");
+    for (int r = 0; r < nrounds && rc == CHY_OK; ++r) {
         std::vector<std::vector<int>> groups;
         for (int i = 0; i < n; ++i) {
             if (round[i] != r) continue;
@@ -550,14 +552,24 @@ int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n)
             groups[k].push_back(i);
         }
         for (size_t k = 0; k < groups.size() && rc == CHY_OK; ++k) {
-            hipStream_t s = ctx->workers[k % nw];
             const std::vector<int> &grp = groups[k];
             const chy_solve_item &lead = items[grp[0]];
+            const int w = grid_index(lead.grid) % nw;
+            hipStream_t s = ctx->workers[w];
+            // order after the latest use of every member grid on another worker
+            std::vector<int> waited;
+            for (int i : grp) {
+                const int v = last_w[grid_index(items[i].grid)];
+                if (v >= 0 && v != w && std::find(waited.begin(), waited.end(), v) == waited.end()) {
+                    rc = dev_rc((int)hipStreamWaitEvent(s, ctx->worker_ev[v], 0), "hipStreamWaitEvent");
+                    waited.push_back(v);
+                }
+            }
             const int np = std::min(lead.npoint, 10);
-            if (grp.size() == 1) {
+            if (rc == CHY_OK && grp.size() == 1) {
                 rc = run_solve(lead.grid, s, lead.maxtrm, lead.isym, lead.redge, np, lead.rho_d, lead.phi_d,
                                lead.coef_d, lead.coef_mode, lead.diag ? lead.diag->klocat : 0);
-            } else {
+            } else if (rc == CHY_OK) {
                 Fields F{};
                 F.nf = (int)grp.size();
                 for (int f = 0; f < F.nf; ++f) {
@@ -573,6 +585,9 @@ int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n)
                                                     hipMemcpyDeviceToDevice, s),
                                 "diagnostics copy");
             }
+            // this worker's progress so far, for later groups that reuse these grids elsewhere
+            if (rc == CHY_OK) rc = dev_rc((int)hipEventRecord(ctx->worker_ev[w], s), "hipEventRecord");
+            for (int i : grp) last_w[grid_index(items[i].grid)] = w;
         }
     }
     // join: the context's stream waits for every worker it forked
