@@ -5,7 +5,8 @@ Workload: the 256x128x256 spiral-arm torus (JMAX=256, KMAX=128, LMAX=256,
 DELR=DELZ=1/256, seeded), plain FP64.  One solve is BDYGEN + POT3 on one
 density field, device resident, COEF for the K=2 plane.  A step is one batch
 of B independent fields (different seeds, each on its own prepared grid)
-solved by one chy_solve_batch call on W worker streams; `value` is solves per
+solved by one chy_solve_batch call, which runs them as batched launches of
+up to 16 fields (default B=16: one launch per kernel); `value` is solves per
 second over the timed steps (B solves per step) with every density already in
 HBM.  B=1 is the single-field solve.
 

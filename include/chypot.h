@@ -139,11 +139,13 @@ int chy_ctx_graphs(chy_ctx *ctx, int enable);
  * One item per field: chy_solve's arguments.  The items run on the context's
  * worker streams, forked from and joined back to the context's stream, so
  * the call is ordered after earlier work on that stream and later work waits
- * for the whole batch.  Items on one grid run in order on one worker (they
- * share the grid's workspace); items on different grids may overlap.  Each
- * item's results equal those of chy_solve on the same arguments.  If any item
- * has a diag pointer the call waits and fills every such diag.  Every item is
- * checked before anything is launched. */
+ * for the whole batch.  Items on one grid object run in order (they share
+ * its workspace).  Items on different grid objects prepared from identical
+ * descriptions, with the same maxtrm, isym, redge, npoint and coef_mode, run
+ * together, up to 16 per launch of each kernel; other items overlap on the
+ * worker streams.  Each item's results are bitwise those of chy_solve on the
+ * same arguments.  If any item has a diag pointer the call waits and fills
+ * every such diag.  Every item is checked before anything is launched. */
 typedef struct {
     chy_grid *grid;
     const double *rho_d;
