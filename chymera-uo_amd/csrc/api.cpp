@@ -284,12 +284,24 @@ static int run_solve(chy_grid *g, hipStream_t s, int maxtrm, int isym, double re
 static int run_solve_fields(chy_grid *g, const Fields &F, hipStream_t s, int maxtrm, int isym, double redge,
                             int npoint, int coef_mode)
 {
+    Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
+    if (p) p->begin(CHY_STAGE_FFT_FWD, s);
     int rc = dev_rc(launch_fft_forward_f(g, F, coef_mode, LANES_INTERIOR, true, true, s), "forward FFT launch");
-    if (!rc)
-        rc = dev_rc(launch_bdygen_from_moments_f(g, F, maxtrm, isym, redge, true, coef_mode, s), "bdygen launch");
-    if (!rc) rc = dev_rc(launch_mode_solve_f(g, F, false, s, nullptr), "mode solve launch");
-    if (!rc) rc = dev_rc(launch_fft_inverse_f(g, F, s), "inverse FFT launch");
-    if (!rc) rc = dev_rc(launch_zaxphi_f(g, F, npoint, s), "ZAXPHI launch");
+    if (p) p->end(s);
+    if (rc) return rc;
+    if (p) p->begin(CHY_STAGE_BDYGEN, s);
+    rc = dev_rc(launch_bdygen_from_moments_f(g, F, maxtrm, isym, redge, true, coef_mode, s), "bdygen launch");
+    if (p) p->end(s);
+    if (rc) return rc;
+    rc = dev_rc(launch_mode_solve_f(g, F, false, s, p), "mode solve launch");
+    if (rc) return rc;
+    if (p) p->begin(CHY_STAGE_FFT_INV, s);
+    rc = dev_rc(launch_fft_inverse_f(g, F, s), "inverse FFT launch");
+    if (p) p->end(s);
+    if (rc) return rc;
+    if (p) p->begin(CHY_STAGE_GHOST, s);
+    rc = dev_rc(launch_zaxphi_f(g, F, npoint, s), "ZAXPHI launch");
+    if (p) p->end(s);
     return rc;
 }
 

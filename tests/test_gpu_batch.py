@@ -97,3 +97,101 @@ def test_batch_rejects_bad_item_before_running(ctx):
     assert float(jobs[0]["phi"].abs().max()) == 0.0   # nothing ran
     for pg in prepared.values():
         pg.close()
+
+
+def _own_grid_job(ctx, seed, shape, coef_mode=chypot.COEF_K2, maxtrm=10, npoint=8, pg=None):
+    """A job on a grid object of its own (or on `pg`), prepared from the
+    description of `shape`; fields on such objects share launches."""
+    g, rho = fields.random_field(seed, tuple((n,) for n in shape))
+    pg = pg if pg is not None else chypot.PreparedGrid(ctx, g)
+    z = dict(dtype=torch.float64, device="cuda")
+    coef = torch.zeros((2, g.lmax // 2, g.kmax + 2, g.jmax + 2), **z) if coef_mode else None
+    return dict(grid=pg, g=g, rho_h=rho, rho=chypot.to_device(rho),
+                phi=torch.zeros((g.lmax, g.kmax + 2, g.jmax + 2), **z), coef=coef,
+                coef_mode=coef_mode, maxtrm=maxtrm, npoint=npoint, redge=0.0)
+
+
+def test_batched_launches_equal_single_solves(ctx):
+    """Fields on separate grid objects with one description run as one launch
+    per kernel (up to 16 fields): 18 such fields (two launch groups), fields of
+    a second shape, a repeat on a grid object already in the batch (a later
+    round), fields whose arguments keep them out of the groups (no COEF,
+    other MAXTRM) and two fields whose NPOINT agree only once lowered to 10 --
+    every one equal bit for bit to its single chy_solve, PHAC included."""
+    print("[gpu] batched launches: 18+3 fields on own grid objects, repeats, odd arguments", flush=True)
+    a, b = (32, 16, 64), (16, 32, 32)
+    jobs = [_own_grid_job(ctx, 300 + i, a) for i in range(18)]
+    jobs += [_own_grid_job(ctx, 400 + i, b) for i in range(3)]
+    jobs.insert(5, _own_grid_job(ctx, 500, a, pg=jobs[0]["grid"]))          # same object: next round
+    jobs.insert(9, _own_grid_job(ctx, 501, a, coef_mode=chypot.COEF_NONE))
+    jobs.insert(13, _own_grid_job(ctx, 502, a, maxtrm=6))
+    jobs.insert(2, _own_grid_job(ctx, 503, a, npoint=12))                   # lowered to 10 ...
+    jobs.insert(17, _own_grid_job(ctx, 504, a, npoint=10))                  # ... so it shares this launch
+    grids = {id(jb["grid"]): jb["grid"] for jb in jobs}
+    try:
+        alone = []
+        for jb in jobs:
+            phi = torch.zeros_like(jb["phi"])
+            coef = torch.zeros_like(jb["coef"]) if jb["coef"] is not None else None
+            d = jb["grid"].solve(jb["rho"], phi, maxtrm=jb["maxtrm"], npoint=jb["npoint"], coef=coef,
+                                 coef_mode=jb["coef_mode"], diag=True)
+            alone.append((phi.cpu(), None if coef is None else coef.cpu(), (d.tmass, d.phichk, d.klocat),
+                          jb["grid"].phac()))
+        for workers in (4, 1):
+            ctx.workers(workers)
+            for jb in jobs:
+                jb["phi"].zero_()
+                if jb["coef"] is not None:
+                    jb["coef"].zero_()
+            diags = ctx.solve_batch(jobs, diag=True)
+            for i, (jb, (phi, coef, d, phac), db) in enumerate(zip(jobs, alone, diags)):
+                assert torch.equal(jb["phi"].cpu(), phi), f"PHI of item {i}, {workers} workers"
+                if coef is not None:
+                    assert torch.equal(jb["coef"].cpu(), coef), f"COEF of item {i}"
+                assert (db.tmass, db.phichk, db.klocat) == d, f"diagnostics of item {i}"
+                if jb["grid"] is not jobs[0]["grid"]:     # PHAC is the grid object's last solve's
+                    assert np.array_equal(jb["grid"].phac(), phac), f"PHAC of item {i}"
+            # asynchronous form: the same numbers once the context's stream is joined
+            for jb in jobs:
+                jb["phi"].zero_()
+            ctx.solve_batch(jobs)
+            ctx.sync()
+            for i, (jb, (phi, _, _, _)) in enumerate(zip(jobs, alone)):
+                assert torch.equal(jb["phi"].cpu(), phi), f"PHI of item {i} (async), {workers} workers"
+        # the launches the batch made: the 18 fields of shape a with NPOINT 8
+        # take two launches (16 + 2), the two with NPOINT 12 (lowered to 10)
+        # and 10 one, the fields of shape b one, the fields without COEF and
+        # with MAXTRM 6 one each, and the repeat on the first grid object one
+        ctx.profile(True)
+        try:
+            for jb in jobs:
+                jb["phi"].zero_()
+            ctx.solve_batch(jobs)
+            ctx.sync()
+            launches = ctx.stage_times()
+        finally:
+            ctx.profile(False)
+        assert launches["fft_fwd"][1] == 7 and launches["ghost"][1] == 7, launches
+        for i, (jb, (phi, _, _, _)) in enumerate(zip(jobs, alone)):
+            assert torch.equal(jb["phi"].cpu(), phi), f"PHI of item {i} (stage timing on)"
+    finally:
+        ctx.workers(4)
+        for pg in grids.values():
+            pg.close()
+
+
+def test_batched_launches_match_oracle(ctx):
+    import pyoracle as po
+    po.use_extended(False)
+    print("[gpu] batched launches: 4 fields on own grid objects vs the oracle", flush=True)
+    jobs = [_own_grid_job(ctx, 600 + i, (32, 16, 64)) for i in range(4)]
+    try:
+        diags = ctx.solve_batch(jobs, diag=True)
+        for jb, d in zip(jobs, diags):
+            want = po.solve(jb["g"], jb["rho_h"])
+            assert rel_max(chypot.to_host(jb["phi"]), want["phi"]) < 1e-12
+            assert abs(d.tmass - want["tmass"]) <= 1e-13 * abs(want["tmass"])
+            assert d.klocat == want["klocat"]
+    finally:
+        for jb in jobs:
+            jb["grid"].close()
