@@ -34,7 +34,7 @@ EXPORTS = (
     "chy_bdygen", "chy_pot3", "chy_solve", "chy_bdygen_host", "chy_pot3_host",
     "chy_ctx_profile", "chy_ctx_stage_times", "chy_ctx_graphs",
     "chy_zaxphi", "chy_zaxphi_host", "chy_phac", "chy_solve_batch", "chy_ctx_workers",
-    "chy_solve_refined",
+    "chy_solve_refined", "chy_source_grav",
     "chy_strerror", "chy_last_device_error",
 )
 
@@ -53,6 +53,12 @@ class GridDesc(C.Structure):
                 ("r", C.POINTER(C.c_double)), ("z", C.POINTER(C.c_double)),
                 ("rhf", C.POINTER(C.c_double)), ("zhf", C.POINTER(C.c_double)),
                 ("dtheta", C.c_double), ("pi", C.c_double), ("grav", C.c_double)]
+
+
+class SourceDesc(C.Structure):
+    _fields_ = [("jmax", C.c_int), ("kmax", C.c_int), ("lmax", C.c_int),
+                ("pot_jmax", C.c_int), ("pot_kmax", C.c_int), ("jstart", C.c_int),
+                ("delt", C.c_double), ("dr", C.c_double), ("dz", C.c_double), ("dtheta", C.c_double)]
 
 
 class Diag(C.Structure):
@@ -101,6 +107,7 @@ def lib():
         L.chy_solve_refined.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp, C.c_int,
                                         C.c_int, C.POINTER(Diag), dp]
         L.chy_ctx_stage_times.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int]
+        L.chy_source_grav.argtypes = [vp, C.POINTER(SourceDesc)] + [vp] * 8
         L.chy_strerror.argtypes = [C.c_int]
         L.chy_strerror.restype = C.c_char_p
         L.chy_last_device_error.restype = C.c_char_p
@@ -169,6 +176,22 @@ class Context:
                                  C.pointer(diags[i]) if diag else None)
         _check(lib().chy_solve_batch(self.handle, items, len(jobs)), "chy_solve_batch")
         return diags
+
+    def source_grav(self, desc: SourceDesc, r, rho, omega, phi, p, s, t, a):
+        """The momentum sources of radhydro/source.f:41-79 (asynchronous on this
+        context's stream).  Device tensors in the Fortran layout: rho, phi of
+        shape (LMAX, POT3KMAX+2, POT3JMAX+2); omega, p, s, t, a of shape
+        (LMAX, KMAX+2, JMAX+2); r of length POT3JMAX+2.  s, t, a are updated."""
+        hyd = (desc.lmax, desc.kmax + 2, desc.jmax + 2)
+        pot = (desc.lmax, desc.pot_kmax + 2, desc.pot_jmax + 2)
+        for name, x, shape in (("rho", rho, pot), ("phi", phi, pot), ("omega", omega, hyd), ("p", p, hyd),
+                               ("s", s, hyd), ("t", t, hyd), ("a", a, hyd)):
+            if tuple(x.shape) != shape:
+                raise ValueError(f"{name}: shape {tuple(x.shape)}, expected {shape}")
+        if r.numel() < desc.pot_jmax + 2:
+            raise ValueError("r: fewer than POT3JMAX+2 values")
+        _check(lib().chy_source_grav(self.handle, C.byref(desc), _dptr(r), _dptr(rho), _dptr(omega), _dptr(phi),
+                                     _dptr(p), _dptr(s), _dptr(t), _dptr(a)), "chy_source_grav")
 
     def stage_times(self):
         """{stage: (total_ms, launches)} since profiling was enabled (waits)."""

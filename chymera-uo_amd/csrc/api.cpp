@@ -850,6 +850,21 @@ int chy_zaxphi_host(chy_grid *g, int npoint, double *phi_h, chy_diag *diag)
     return diag ? read_diag(g, diag, false, true) : CHY_OK;
 }
 
+int chy_source_grav(chy_ctx *ctx, const chy_source_desc *d, const double *r_d, const double *rho_d,
+                    const double *omega_d, const double *phi_d, const double *p_d, double *s_d,
+                    double *t_d, double *a_d)
+{
+    if (!ctx) return CHY_ESTATE;
+    if (!d || !r_d || !rho_d || !omega_d || !phi_d || !p_d || !s_d || !t_d || !a_d) return CHY_EINVAL;
+    if (d->jmax < 1 || d->kmax < 1 || d->lmax < 1 || d->pot_jmax < d->jmax || d->pot_kmax < d->kmax)
+        return CHY_EINVAL;
+    if (d->jstart < 2 || d->jstart > d->jmax + 1) return CHY_EINVAL;
+    if (!(d->dr > 0.0) || !(d->dz > 0.0) || !(d->dtheta > 0.0)) return CHY_EINVAL;
+    DeviceScope ds(ctx->device);
+    return dev_rc(launch_source_grav(d, r_d, rho_d, omega_d, phi_d, p_d, s_d, t_d, a_d, ctx->stream),
+                  "SOURCE launch");
+}
+
 int chy_phac(chy_grid *g, double *phac_h, int n)
 {
     if (!g) return CHY_ESTATE;

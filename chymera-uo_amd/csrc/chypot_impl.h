@@ -250,6 +250,9 @@ int launch_bdygen_from_moments(const chy_grid *g, int maxtrm, int isym, double r
 int launch_bdygen_from_moments_f(const chy_grid *g, const Fields &F, int maxtrm, int isym, double redge,
                                  bool spectral, int coef_mode, hipStream_t s);
 bool bdygen_spectral_ok(const chy_grid *g);
+// The momentum sources of radhydro/source.f:41-79 (arguments checked by the caller).
+int launch_source_grav(const chy_source_desc *d, const double *r, const double *rho, const double *omega,
+                       const double *phi, const double *p, double *s, double *t, double *a, hipStream_t st);
 
 void set_device_error(hipError_t e, const char *where);
 

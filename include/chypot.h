@@ -195,6 +195,29 @@ int chy_zaxphi_host(chy_grid *g, int npoint, double *phi_h, chy_diag *diag);
  * IPRINT=1 report of pot3.f:402-404. */
 int chy_phac(chy_grid *g, double *phac_h, int n);
 
+/* ---- momentum sources that read the potential (radhydro/source.f:41-79) ----
+ * The first loop of SOURCE with BINARY=0, which the hydro step runs right
+ * after BDYGEN+POT3 (radhydro/3dhyd-main.f:475-477): for K=2..KMAX+1,
+ * L=1..LMAX (periodic), S at J=JSTART+1..JMAX+1 and T, A at J=JSTART..JMAX+1
+ * get DELT times the centrifugal, gravity (PHI) and pressure terms, in the
+ * reference's order of operations (bitwise the reference's results).
+ * Device arrays in Fortran order: rho_d, phi_d (POT3JMAX+2, POT3KMAX+2, LMAX)
+ * as in /POIS/; omega_d, p_d, s_d, t_d, a_d (JMAX+2, KMAX+2, LMAX) as in /EOM/
+ * and /STATES/; r_d is R(1..POT3JMAX+2) of /GRID/.  s_d, t_d and a_d are
+ * updated in place; no other cell is written.  Asynchronous on the context's
+ * stream.  CHY_EINVAL for null pointers, a hydro grid larger than the
+ * potential grid, JSTART outside 2..JMAX+1, or a spacing that is not > 0. */
+typedef struct {
+    int jmax, kmax, lmax;   /* hydro grid (JMAX, KMAX, LMAX) */
+    int pot_jmax, pot_kmax; /* potential grid (POT3JMAX, POT3KMAX) */
+    int jstart;             /* max(JMIN, 2) (source.f:31-35) */
+    double delt;            /* DELT */
+    double dr, dz, dtheta;  /* ROF3N, ZOF3N, DTHETA */
+} chy_source_desc;
+int chy_source_grav(chy_ctx *ctx, const chy_source_desc *d, const double *r_d, const double *rho_d,
+                    const double *omega_d, const double *phi_d, const double *p_d, double *s_d,
+                    double *t_d, double *a_d);
+
 /* ---- stage timing (no reference counterpart; for benchmarks) ----
  * While enabled, every stage of chy_bdygen/chy_pot3/chy_solve is bracketed by
  * HIP events on the context's stream.  chy_ctx_stage_times() waits for the

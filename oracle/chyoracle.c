@@ -864,3 +864,56 @@ int ora_residual(int jmax, int kmax, int lmax, const double *rhf, const double *
     pot3_coef_free(&pc);
     return 0;
 }
+
+/*
+ * Momentum sources that read the potential: the first loop of SOURCE
+ * (radhydro/source.f:41-79, BINARY=0) with the loop bounds of source.f:41-48
+ * (L periodic, K=2..KMAX+1), S from J=JSTART+1 (source.f:51), T and A from
+ * J=JSTART (source.f:63,70), JSTART = max(JMIN, 2) (source.f:31-35).
+ * Always binary64 (also in the extended build), each operation in the order
+ * the Fortran expressions give: parenthesised sums first, then products and
+ * quotients left to right (source.f:52-56, 64-66, 71-72).  x86-64 without FMA
+ * rounds every operation separately, as the reference build does.
+ * Arrays are Fortran-ordered; rho and phi have the potential grid's leading
+ * dimensions (pjmax+2, pkmax+2), the others the hydro grid's (jmax+2, kmax+2).
+ */
+int ora_source_grav(int jmax, int kmax, int lmax, int pjmax, int pkmax, int jstart, double delt,
+                    double dr, double dz, double dtheta, const double *r, const double *rho,
+                    const double *omega, const double *phi, const double *p, double *s, double *t,
+                    double *a)
+{
+    if (jmax < 1 || kmax < 1 || lmax < 1 || pjmax < jmax || pkmax < kmax || jstart < 2 || jstart > jmax + 1)
+        return 1;
+    const long hj = jmax + 2, hk = kmax + 2, pj = pjmax + 2, pk = pkmax + 2;
+#define HX(J, K, L) (((long)(L) - 1) * hk * hj + ((long)(K) - 1) * hj + (long)(J) - 1)
+#define PX(J, K, L) (((long)(L) - 1) * pk * pj + ((long)(K) - 1) * pj + (long)(J) - 1)
+    for (int L = 1; L <= lmax; ++L) {
+        int LM = L - 1, LP = L + 1;
+        if (L == 1) LM = lmax;
+        if (L == lmax) LP = 1;
+        for (int K = 2; K <= kmax + 1; ++K) {
+            for (int J = jstart + 1; J <= jmax + 1; ++J) {
+                double rr = rho[PX(J, K, L)] + rho[PX(J - 1, K, L)];
+                double oo = omega[HX(J, K, L)] + omega[HX(J - 1, K, L)];
+                double t1 = r[J - 1] * rr * (oo * oo) * 0.25 * 0.5;
+                double t2 = rr * (phi[PX(J, K, L)] - phi[PX(J - 1, K, L)]) / (2.0 * dr);
+                double t3 = (p[HX(J, K, L)] - p[HX(J - 1, K, L)]) / dr;
+                s[HX(J, K, L)] = s[HX(J, K, L)] + delt * (t1 - t2 - t3);
+            }
+            for (int J = jstart; J <= jmax + 1; ++J) {
+                double t1 = 0.5 * (rho[PX(J, K, L)] + rho[PX(J, K - 1, L)]) *
+                            (phi[PX(J, K, L)] - phi[PX(J, K - 1, L)]) / dz;
+                double t2 = (p[HX(J, K, L)] - p[HX(J, K - 1, L)]) / dz;
+                t[HX(J, K, L)] = t[HX(J, K, L)] - delt * (t1 + t2);
+            }
+            for (int J = jstart; J <= jmax + 1; ++J) {
+                double x = rho[PX(J, K, L)] * (phi[PX(J, K, LP)] - phi[PX(J, K, LM)]) + p[HX(J, K, LP)] -
+                           p[HX(J, K, LM)];
+                a[HX(J, K, L)] = a[HX(J, K, L)] - delt * x / (2.0 * dtheta);
+            }
+        }
+    }
+#undef HX
+#undef PX
+    return 0;
+}

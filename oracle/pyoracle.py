@@ -42,6 +42,7 @@ def lib():
         L.ora_residual.argtypes = [C.c_int, C.c_int, C.c_int, d, d, C.c_double, C.c_double,
                                    C.c_double, d, d, d, d]
         L.ora_terms.argtypes = [i, i, d, i, d]
+        L.ora_source_grav.argtypes = [C.c_int] * 6 + [C.c_double] * 4 + [d] * 8
         _libs[path] = L
     return _libs[path]
 
@@ -152,6 +153,23 @@ def residual(g, phi, rho):
     lib().ora_residual(g.jmax, g.kmax, g.lmax, _p(rhf), _p(zhf), g.dtheta, 3.1415926535897932, 1.0,
                        _p(phi), _p(rho), _p(res), _p(scale))
     return res, scale
+
+
+def source_grav(jmax, kmax, lmax, pjmax, pkmax, jstart, delt, dr, dz, dtheta, r, rho, omega, phi, p, s, t, a):
+    """The first loop of SOURCE (radhydro/source.f:41-79); s, t, a (Fortran-
+    ordered float64 arrays of shape (JMAX+2, KMAX+2, LMAX)) are updated in
+    place; rho, phi have shape (PJMAX+2, PKMAX+2, LMAX), r length PJMAX+2."""
+    hyd, pot = (jmax + 2, kmax + 2, lmax), (pjmax + 2, pkmax + 2, lmax)
+    args = []
+    for x, shape in ((r, None), (rho, pot), (omega, hyd), (phi, pot), (p, hyd), (s, hyd), (t, hyd), (a, hyd)):
+        if shape is not None and (x.shape != shape or not x.flags.f_contiguous or x.dtype != np.float64):
+            raise ValueError(f"expected a Fortran-ordered float64 array of shape {shape}")
+        args.append(x)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    args[0] = r
+    rc = lib().ora_source_grav(jmax, kmax, lmax, pjmax, pkmax, jstart, delt, dr, dz, dtheta, *[_p(x) for x in args])
+    if rc:
+        raise ValueError("ora_source_grav: bad arguments")
 
 
 def set_threads(n: int):
