@@ -20,7 +20,10 @@ The line also carries:
     stage on the solve's stream; `roofline.traffic` is the HBM bytes per launch
     that rocprofv3 FETCH_SIZE/WRITE_SIZE recorded for that kernel
     (profiles/potsolve_256x128x256_hbm_traffic.txt), or null;
-  * the reference CPU solve timed on this host.
+  * the reference CPU solve timed on this host;
+  * `source_grav`: the hydro step's momentum sources that read PHI
+    (radhydro/source.f:41-79) on the same grid, per call, with the oracle's
+    C loop timed beside it (--source-calls 0 skips it).
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--workers W] [--gpus 1]
 """
@@ -141,6 +144,52 @@ def serve_run(ctx, args):
     return out
 
 
+def source_row(ctx, stream, g, n, with_cpu):
+    """The momentum sources of radhydro/source.f:41-79 on the solve's grid
+    (hydro grid = potential grid, JSTART=2): per-call time from HIP events on
+    the solve stream around n calls; algorithmic bytes are 80 per cell of the
+    T/A range (RHO, OMEGA, PHI, P read; S, T, A read and written); with_cpu
+    also times the oracle's C loop on this host (1 thread) on the same data."""
+    import torch
+
+    import chypot
+    rng = np.random.default_rng(7)
+    shape = (g.lmax, g.kmax + 2, g.jmax + 2)
+    host = [rng.standard_normal(shape) for _ in range(7)]
+    rho, om, phi, p, s, t, a = (torch.from_numpy(x).to("cuda") for x in host)
+    r = torch.from_numpy(g.r()).to("cuda")
+    d = chypot.SourceDesc(g.jmax, g.kmax, g.lmax, g.jmax, g.kmax, 2, 1.0e-3, g.delr, g.delz, g.dtheta)
+    for _ in range(3):
+        ctx.source_grav(d, r, rho, om, phi, p, s, t, a)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(n):
+        ctx.source_grav(d, r, rho, om, phi, p, s, t, a)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / n
+    cells = (g.jmax + 1 - d.jstart + 1) * g.kmax * g.lmax
+    nbytes = 80 * cells
+    out = {"kernel": "k_source_grav", "grid": g.name, "calls": n, "us_per_call": ms * 1e3,
+           "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / (ms * 1e-3) / 1e9,
+           "frac_hbm": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    del rho, om, phi, p, s, t, a
+    if with_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pyoracle
+        f = [np.asfortranarray(x.transpose(2, 1, 0)) for x in host]
+        secs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            pyoracle.source_grav(g.jmax, g.kmax, g.lmax, g.jmax, g.kmax, 2, d.delt, d.dr, d.dz, d.dtheta,
+                                 g.r(), f[0], f[1], f[2], f[3], f[4], f[5], f[6])
+            secs.append(time.perf_counter() - t0)
+        out["cpu_oracle_ms"] = float(np.median(secs)) * 1e3
+        out["cpu_cores"] = 1
+    return out
+
+
 def cpu_baseline(g, rho, nrep):
     """Reference BDYGEN+POT3 (flang -O3 real-8 build of pot/*.f, oracle/_ref) on
     this host, 1 thread; falls back to the oracle's C restatement."""
@@ -198,6 +247,8 @@ def main():
                          "(60%% 128x64x128, 30%% 256x128x256, 10%% 512x256x512)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="arrivals per second")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="HBM budget of the server")
+    ap.add_argument("--source-calls", type=int, default=20,
+                    help="also time this many calls of the SOURCE momentum sources (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the solves, copy a buffer of this many MiB device to device "
                          "three times (a kernel of known traffic for scaling HBM counters)")
@@ -338,6 +389,8 @@ def main():
     roof["traffic"] = hbm_traffic(dom)
     hetero = hetero_batch(ctx, args.hetero) if args.hetero > 0 else None
     serving = serve_run(ctx, args) if args.serve > 0 else None
+    source = (source_row(ctx, stream, g, args.source_calls, not args.no_cpu_baseline)
+              if args.source_calls > 0 else None)
     cpu = None if args.no_cpu_baseline else cpu_baseline(g, rho, args.cpu_reps)
     out = {
         "metric": METRIC,
@@ -365,6 +418,7 @@ def main():
         "refined_path": refined_line,
         "hetero_batch": hetero,
         "serving": serving,
+        "source_grav": source,
         "stages_ms": {k: round(v, 5) for k, v in stage_avg.items()},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -15,11 +15,12 @@
 // in the order the Fortran expressions give, one rounding each (no fused
 // multiply-add), so the result is the reference's to the bit.
 //
-// One thread per (J, K, L), J fastest: a wavefront reads 64 consecutive J of
-// each array.  Blocks of 64 J x 4 K walk one L plane after another, so the
-// plane L-1 that a block needs was read by the block with the same (J, K)
-// tile of the previous plane -- on the same XCD, since a plane has a
-// multiple of 8 blocks whenever KMAX is a multiple of 32 -- and comes from L2.
+// One thread per (J, L) and run of SRC_KC rows K, J fastest: a wavefront
+// reads 64 consecutive J of each array.  The thread walks its rows upward and
+// carries RHO, PHI and P of row K-1 in registers; a block covers SRC_LB
+// consecutive planes, so the L+-1 neighbours of all but its two edge planes
+// are elements its own wavefronts read in the same step, and the J-1
+// neighbour is the adjacent lane's.
 #include "chypot_impl.h"
 
 namespace chy {
@@ -34,38 +35,110 @@ struct SourceArgs {
     double delt, dr, dz, dtheta;
 };
 
-__global__ __launch_bounds__(256) void k_source_grav(SourceArgs a)
+// Raw buffer stores (resource word 3 of the gfx9 family): a lane whose
+// offset is kDropOffset, beyond the range, has its store dropped, so the
+// conditional S store needs no branch.
+constexpr int kRawRange = 0x7fffffff;
+constexpr int kRawWord3 = 0x00020000;
+constexpr unsigned kDropOffset = 0x80000000u;
+typedef unsigned int ux2 __attribute__((ext_vector_type(2)));
+
+constexpr int SRC_KC = 16;  // K rows per thread
+constexpr int SRC_LB = 8;   // L planes per block
+
+__global__ __launch_bounds__(64 * SRC_LB) void k_source_grav(SourceArgs a, int nbx, int nby, int nblocks)
 {
 #pragma clang fp contract(off)
-    const int J = a.jstart + (int)(blockIdx.x * 64 + threadIdx.x);  // Fortran indices
-    const int K = 2 + (int)(blockIdx.y * 4 + threadIdx.y);
-    const int L = 1 + (int)blockIdx.z;
-    if (J > a.jmax1 || K > a.kmax + 1) return;
+    // Blocks are handed to the 8 XCDs in turn (block b to XCD b % 8); tile
+    // t = (b % 8) * (nblocks / 8) + b / 8 puts consecutive tiles -- the J
+    // neighbours, then the L neighbours, whose rows share cache lines and
+    // planes -- on one XCD and so behind one L2.
+    const int b = (int)blockIdx.x;
+    const int tile = nblocks % 8 == 0 ? (b % 8) * (nblocks / 8) + b / 8 : b;
+    const int bx = tile % nbx, by = (tile / nbx) % nby, bz = tile / (nbx * nby);
+    const int J = a.jstart + bx * 64 + (int)threadIdx.x;  // Fortran indices
+    const int L = 1 + by * SRC_LB + (int)threadIdx.y;
+    const int K0 = 2 + bz * SRC_KC;
+    if (J > a.jmax1 || L > a.lmax) return;
+    const int K1 = min(K0 + SRC_KC - 1, a.kmax + 1);
     const int LM = L == 1 ? a.lmax : L - 1;
     const int LP = L == a.lmax ? 1 : L + 1;
     const size_t hp = (size_t)a.j2 * a.k2, pp = (size_t)a.pj2 * a.pk2;
-    const size_t h = (size_t)(J - 1) + (size_t)a.j2 * (K - 1) + hp * (L - 1);
-    const size_t q = (size_t)(J - 1) + (size_t)a.pj2 * (K - 1) + pp * (L - 1);
-    const double rho = a.rho[q], phi = a.phi[q], p = a.p[h];
+    const size_t hl = (size_t)(J - 1) + hp * (L - 1);  // this (J, L) at K = 1
+    const size_t ql = (size_t)(J - 1) + pp * (L - 1);
+    const ptrdiff_t hdp = (ptrdiff_t)hp * (LP - L), hdm = (ptrdiff_t)hp * (L - LM);
+    const ptrdiff_t qdp = (ptrdiff_t)pp * (LP - L), qdm = (ptrdiff_t)pp * (L - LM);
+    const bool sj = J >= a.jstart + 1;
+    const double twodr = 2.0 * a.dr, twodth = 2.0 * a.dtheta;
+    // the arrays do not overlap: every load of a row can be issued before
+    // its stores
+    const double *__restrict__ rr = a.r;
+    const double *__restrict__ rho_ = a.rho;
+    const double *__restrict__ om_ = a.omega;
+    const double *__restrict__ phi_ = a.phi;
+    const double *__restrict__ p_ = a.p;
+    double *__restrict__ s_ = a.s;
+    double *__restrict__ t_ = a.t;
+    double *__restrict__ a_ = a.a;
 
-    if (J >= a.jstart + 1) {
-        const double rs = rho + a.rho[q - 1];
-        const double om = a.omega[h] + a.omega[h - 1];
-        const double cen = a.r[J - 1] * rs * (om * om) * 0.25 * 0.5;
-        const double grav = rs * (phi - a.phi[q - 1]) / (2.0 * a.dr);
-        const double pres = (p - a.p[h - 1]) / a.dr;
-        a.s[h] = a.s[h] + a.delt * (cen - grav - pres);
-    }
-    {
-        const double grav = 0.5 * (rho + a.rho[q - a.pj2]) * (phi - a.phi[q - a.pj2]) / a.dz;
-        const double pres = (p - a.p[h - a.j2]) / a.dz;
-        a.t[h] = a.t[h] - a.delt * (grav + pres);
-    }
-    {
-        const size_t qp = q + pp * (size_t)(LP - L), qm = q - pp * (size_t)(L - LM);
-        const size_t hpl = h + hp * (size_t)(LP - L), hml = h - hp * (size_t)(L - LM);
-        const double x = rho * (a.phi[qp] - a.phi[qm]) + a.p[hpl] - a.p[hml];
-        a.a[h] = a.a[h] - a.delt * x / (2.0 * a.dtheta);
+    const double rj = rr[J - 1];
+    // S stores through a buffer over this wavefront's plane (a wavefront is
+    // 64 J of one plane L); the offset stays below the plane size
+    const int Lw = __builtin_amdgcn_readfirstlane(L);
+    const __amdgpu_buffer_rsrc_t srs = __builtin_amdgcn_make_buffer_rsrc(s_ + hp * (size_t)(Lw - 1), 0, kRawRange,
+                                                                        kRawWord3);
+    // Everything one row K reads; the next row's values are loaded before
+    // this row's stores are issued, so the loads of consecutive rows overlap
+    // (the last row loads itself again instead of a row beyond the run).
+    struct Row {
+        double rho, phi, p, rhoj, phij, pj, om0, omj, phip, phim, ppl, pml, s0, t0, a0;
+    };
+    auto load = [&](int K) {
+        const size_t h = hl + (size_t)a.j2 * (K - 1), q = ql + (size_t)a.pj2 * (K - 1);
+        Row w;
+        w.rho = rho_[q]; w.phi = phi_[q]; w.p = p_[h];
+        w.rhoj = rho_[q - 1]; w.phij = phi_[q - 1]; w.pj = p_[h - 1];  // in bounds for J = JSTART too
+        w.om0 = om_[h]; w.omj = om_[h - 1];
+        w.phip = phi_[q + qdp]; w.phim = phi_[q - qdm]; w.ppl = p_[h + hdp]; w.pml = p_[h - hdm];
+        w.s0 = s_[h]; w.t0 = t_[h]; w.a0 = a_[h];
+        return w;
+    };
+    // RHO, PHI and P of row K-1, carried from one row to the next
+    double rhom = rho_[ql + (size_t)a.pj2 * (K0 - 2)], phm = phi_[ql + (size_t)a.pj2 * (K0 - 2)];
+    double pm = p_[hl + (size_t)a.j2 * (K0 - 2)];
+    auto step = [&](int K, const Row &c) {
+        const size_t h = hl + (size_t)a.j2 * (K - 1);
+        {
+            const double rs = c.rho + c.rhoj;
+            const double om = c.om0 + c.omj;
+            const double cen = rj * rs * (om * om) * 0.25 * 0.5;
+            const double grav = rs * (c.phi - c.phij) / twodr;
+            const double pres = (c.p - c.pj) / a.dr;
+            const unsigned vo = sj ? (unsigned)((h - hp * (size_t)(L - 1)) * sizeof(double)) : kDropOffset;
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(ux2, c.s0 + a.delt * (cen - grav - pres)), srs,
+                                                  vo, 0, 0);
+        }
+        {
+            const double grav = 0.5 * (c.rho + rhom) * (c.phi - phm) / a.dz;
+            const double pres = (c.p - pm) / a.dz;
+            t_[h] = c.t0 - a.delt * (grav + pres);
+        }
+        {
+            const double x = c.rho * (c.phip - c.phim) + c.ppl - c.pml;
+            a_[h] = c.a0 - a.delt * x / twodth;
+        }
+        rhom = c.rho; phm = c.phi; pm = c.p;
+    };
+    // two register sets in turn: the row after next loads while a row is
+    // computed and stored
+    Row c0 = load(K0);
+    for (int K = K0;; K += 2) {
+        const Row c1 = load(min(K + 1, K1));
+        step(K, c0);
+        if (K + 1 > K1) break;
+        c0 = load(min(K + 2, K1));
+        step(K + 1, c1);
+        if (K + 2 > K1) break;
     }
 }
 
@@ -82,8 +155,9 @@ int launch_source_grav(const chy_source_desc *d, const double *r, const double *
     x.jstart = d->jstart; x.jmax1 = d->jmax + 1; x.kmax = d->kmax; x.lmax = d->lmax;
     x.delt = d->delt; x.dr = d->dr; x.dz = d->dz; x.dtheta = d->dtheta;
     const int nj = x.jmax1 - x.jstart + 1;
-    const dim3 grid((nj + 63) / 64, (d->kmax + 3) / 4, d->lmax), block(64, 4);
-    hipLaunchKernelGGL(k_source_grav, grid, block, 0, st, x);
+    const int nbx = (nj + 63) / 64, nby = (d->lmax + SRC_LB - 1) / SRC_LB, nbz = (d->kmax + SRC_KC - 1) / SRC_KC;
+    const int nblocks = nbx * nby * nbz;
+    hipLaunchKernelGGL(k_source_grav, dim3(nblocks), dim3(64, SRC_LB), 0, st, x, nbx, nby, nblocks);
     return (int)hipGetLastError();
 }
 
