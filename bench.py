@@ -23,7 +23,9 @@ The line also carries:
   * the reference CPU solve timed on this host;
   * `source_grav`: the hydro step's momentum sources that read PHI
     (radhydro/source.f:41-79) on the same grid, per call, with the oracle's
-    C loop timed beside it (--source-calls 0 skips it).
+    C loop timed beside it, and the star potential added to PHI before them
+    (pot/ExternalPot.f, `source_grav.external_pot`) (--source-calls 0 skips
+    both).
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--workers W] [--gpus 1]
 """
@@ -174,6 +176,22 @@ def source_row(ctx, stream, g, n, with_cpu):
     out = {"kernel": "k_source_grav", "grid": g.name, "calls": n, "us_per_call": ms * 1e3,
            "algorithmic_bytes": nbytes, "achieved_GBs": nbytes / (ms * 1e-3) / 1e9,
            "frac_hbm": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    # the star potential of ExternalPot on the same PHI: 24 bytes per cell
+    # (PHI read and written, STARPHI written)
+    e = chypot.ExtPotDesc(g.jmax, g.kmax, g.lmax, g.jmax, g.kmax, g.dtheta, 1.0, 0.0, 0.02, 0.3)
+    rhf, zhf = torch.from_numpy(g.rhf()).to("cuda"), torch.from_numpy(g.zhf()).to("cuda")
+    ctx.external_pot(e, rhf, zhf, phi, s)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(n):
+        ctx.external_pot(e, rhf, zhf, phi, s)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_e = ev0.elapsed_time(ev1) / n
+    eb = 24 * (g.jmax + 2) * (g.kmax + 2) * g.lmax
+    out["external_pot"] = {"kernel": "k_external_pot", "us_per_call": ms_e * 1e3, "algorithmic_bytes": eb,
+                           "achieved_GBs": eb / (ms_e * 1e-3) / 1e9,
+                           "frac_hbm": eb / (ms_e * 1e-3) / 1e9 / HBM_PEAK_GBS}
     del rho, om, phi, p, s, t, a
     if with_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))

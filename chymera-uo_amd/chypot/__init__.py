@@ -34,7 +34,7 @@ EXPORTS = (
     "chy_bdygen", "chy_pot3", "chy_solve", "chy_bdygen_host", "chy_pot3_host",
     "chy_ctx_profile", "chy_ctx_stage_times", "chy_ctx_graphs",
     "chy_zaxphi", "chy_zaxphi_host", "chy_phac", "chy_solve_batch", "chy_ctx_workers",
-    "chy_solve_refined", "chy_source_grav",
+    "chy_solve_refined", "chy_source_grav", "chy_external_pot",
     "chy_strerror", "chy_last_device_error",
 )
 
@@ -59,6 +59,13 @@ class SourceDesc(C.Structure):
     _fields_ = [("jmax", C.c_int), ("kmax", C.c_int), ("lmax", C.c_int),
                 ("pot_jmax", C.c_int), ("pot_kmax", C.c_int), ("jstart", C.c_int),
                 ("delt", C.c_double), ("dr", C.c_double), ("dz", C.c_double), ("dtheta", C.c_double)]
+
+
+class ExtPotDesc(C.Structure):
+    _fields_ = [("jmax", C.c_int), ("kmax", C.c_int), ("lmax", C.c_int),
+                ("pot_jmax", C.c_int), ("pot_kmax", C.c_int), ("dtheta", C.c_double),
+                ("mass_star", C.c_double), ("tmassacc", C.c_double), ("rpstar", C.c_double),
+                ("phi_star", C.c_double)]
 
 
 class Diag(C.Structure):
@@ -108,6 +115,7 @@ def lib():
                                         C.c_int, C.POINTER(Diag), dp]
         L.chy_ctx_stage_times.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int]
         L.chy_source_grav.argtypes = [vp, C.POINTER(SourceDesc)] + [vp] * 8
+        L.chy_external_pot.argtypes = [vp, C.POINTER(ExtPotDesc)] + [vp] * 4
         L.chy_strerror.argtypes = [C.c_int]
         L.chy_strerror.restype = C.c_char_p
         L.chy_last_device_error.restype = C.c_char_p
@@ -192,6 +200,19 @@ class Context:
             raise ValueError("r: fewer than POT3JMAX+2 values")
         _check(lib().chy_source_grav(self.handle, C.byref(desc), _dptr(r), _dptr(rho), _dptr(omega), _dptr(phi),
                                      _dptr(p), _dptr(s), _dptr(t), _dptr(a)), "chy_source_grav")
+
+    def external_pot(self, desc: ExtPotDesc, rhf, zhf, phi, starphi):
+        """The star potential of pot/ExternalPot.f (INDIRECT=0, BINARY=0), asynchronous on
+        this context's stream: starphi (LMAX, KMAX+2, JMAX+2) is written and added to
+        phi (LMAX, POT3KMAX+2, POT3JMAX+2); rhf, zhf are device vectors of RHF, ZHF."""
+        if tuple(phi.shape) != (desc.lmax, desc.pot_kmax + 2, desc.pot_jmax + 2):
+            raise ValueError(f"phi: shape {tuple(phi.shape)}")
+        if tuple(starphi.shape) != (desc.lmax, desc.kmax + 2, desc.jmax + 2):
+            raise ValueError(f"starphi: shape {tuple(starphi.shape)}")
+        if rhf.numel() < desc.jmax + 2 or zhf.numel() < desc.kmax + 2:
+            raise ValueError("rhf/zhf: too few values")
+        _check(lib().chy_external_pot(self.handle, C.byref(desc), _dptr(rhf), _dptr(zhf), _dptr(phi),
+                                      _dptr(starphi)), "chy_external_pot")
 
     def stage_times(self):
         """{stage: (total_ms, launches)} since profiling was enabled (waits)."""

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -469,6 +470,7 @@ int chy_ctx_destroy(chy_ctx *ctx)
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->d_diag) (void)hipFree(ctx->d_diag);
     if (ctx->h_diag) (void)hipHostFree(ctx->h_diag);
+    for (const chy_ctx::TrigTable &t : ctx->trig) (void)hipFree(t.d);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
     return CHY_OK;
@@ -863,6 +865,43 @@ int chy_source_grav(chy_ctx *ctx, const chy_source_desc *d, const double *r_d, c
     DeviceScope ds(ctx->device);
     return dev_rc(launch_source_grav(d, r_d, rho_d, omega_d, phi_d, p_d, s_d, t_d, a_d, ctx->stream),
                   "SOURCE launch");
+}
+
+int chy_external_pot(chy_ctx *ctx, const chy_extpot_desc *d, const double *rhf_d, const double *zhf_d,
+                     double *phi_d, double *starphi_d)
+{
+    if (!ctx) return CHY_ESTATE;
+    if (!d || !rhf_d || !zhf_d || !phi_d || !starphi_d) return CHY_EINVAL;
+    if (d->jmax < 1 || d->kmax < 1 || d->lmax < 1 || d->pot_jmax < d->jmax || d->pot_kmax < d->kmax)
+        return CHY_EINVAL;
+    if (!(d->dtheta > 0.0)) return CHY_EINVAL;
+    DeviceScope ds(ctx->device);
+    const double *cs = nullptr;
+    for (const chy_ctx::TrigTable &t : ctx->trig)
+        if (t.lmax == d->lmax && t.dtheta == d->dtheta) cs = t.d;
+    if (!cs) {
+        // psi_L = (L-1+1/2) DTHETA and its cos/sin as ExternalPot.f:46-48 forms them
+        std::vector<double> h(2 * (size_t)d->lmax);
+        for (int l = 1; l <= d->lmax; ++l) {
+            const double psi = ((double)(l - 1) + 0.5) * d->dtheta;
+            h[2 * (l - 1)] = std::cos(psi);
+            h[2 * (l - 1) + 1] = std::sin(psi);
+        }
+        double *dev = nullptr;
+        if (hipMalloc(&dev, h.size() * sizeof(double)) != hipSuccess) return CHY_ENOMEM;
+        int rc = dev_rc((int)hipMemcpy(dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice),
+                        "cos/sin table upload");
+        if (rc) {
+            (void)hipFree(dev);
+            return rc;
+        }
+        ctx->trig.push_back(chy_ctx::TrigTable{d->lmax, d->dtheta, dev});
+        cs = dev;
+    }
+    // the star's position as ExternalPot.f:18-19 forms it
+    const double ix = d->rpstar * std::cos(d->phi_star), iy = d->rpstar * std::sin(d->phi_star);
+    return dev_rc(launch_external_pot(d, rhf_d, zhf_d, cs, phi_d, starphi_d, ix, iy, ctx->stream),
+                  "ExternalPot launch");
 }
 
 int chy_phac(chy_grid *g, double *phac_h, int n)

@@ -126,6 +126,13 @@ struct chy_ctx {
     hipEvent_t fork_ev = nullptr;
     double *d_diag = nullptr, *h_diag = nullptr;
     int diag_cap = 0;
+    // chy_external_pot: (cos psi_L, sin psi_L) tables per (LMAX, DTHETA)
+    struct TrigTable {
+        int lmax;
+        double dtheta;
+        double *d;
+    };
+    std::vector<TrigTable> trig;
 };
 
 namespace chy {
@@ -250,6 +257,10 @@ int launch_bdygen_from_moments(const chy_grid *g, int maxtrm, int isym, double r
 int launch_bdygen_from_moments_f(const chy_grid *g, const Fields &F, int maxtrm, int isym, double redge,
                                  bool spectral, int coef_mode, hipStream_t s);
 bool bdygen_spectral_ok(const chy_grid *g);
+// The star potential of pot/ExternalPot.f (arguments checked by the caller; cs
+// holds (cos psi_L, sin psi_L) for L = 1..LMAX, ix/iy the star's position).
+int launch_external_pot(const chy_extpot_desc *d, const double *rhf, const double *zhf, const double *cs,
+                        double *phi, double *starphi, double ix, double iy, hipStream_t st);
 // The momentum sources of radhydro/source.f:41-79 (arguments checked by the caller).
 int launch_source_grav(const chy_source_desc *d, const double *r, const double *rho, const double *omega,
                        const double *phi, const double *p, double *s, double *t, double *a, hipStream_t st);

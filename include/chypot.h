@@ -218,6 +218,29 @@ int chy_source_grav(chy_ctx *ctx, const chy_source_desc *d, const double *r_d, c
                     const double *omega_d, const double *phi_d, const double *p_d, double *s_d,
                     double *t_d, double *a_d);
 
+/* ---- the central star's potential (pot/ExternalPot.f:1-88) ----
+ * ExternalPot as the default build runs it (EXTERNAL_POT=1, INDIRECT=0,
+ * BINARY=0), which the hydro step calls after POT3
+ * (radhydro/3dhyd-main.f:494-511): with the star at (RPSTAR cos PHI_STAR,
+ * RPSTAR sin PHI_STAR), STARPHI(J,K,L) = -(MASS_STAR+TMASSACC)/|x - x*| at
+ * (RHF(J), ZHF(K), psi_L = (L-1/2) DTHETA) for J=1..JMAX+2, K=1..KMAX+2,
+ * L=1..LMAX, and PHI += STARPHI there (bitwise the reference's arithmetic).
+ * phi_d has the potential grid's layout (POT3JMAX+2, POT3KMAX+2, LMAX), starphi_d
+ * (/GAP/) the hydro grid's (JMAX+2, KMAX+2, LMAX); rhf_d, zhf_d are RHF, ZHF of
+ * /GRID/ (at least JMAX+2 and KMAX+2 values).  The first call for an LMAX and
+ * DTHETA builds a small cos/sin table (waits once).  Asynchronous on the
+ * context's stream otherwise.  CHY_EINVAL for null pointers, a hydro grid
+ * larger than the potential grid, or DTHETA not > 0. */
+typedef struct {
+    int jmax, kmax, lmax;   /* hydro grid (JMAX, KMAX, LMAX) */
+    int pot_jmax, pot_kmax; /* potential grid (POT3JMAX, POT3KMAX) */
+    double dtheta;          /* DTHETA */
+    double mass_star, tmassacc;
+    double rpstar, phi_star; /* the star's position (/movingstar/) */
+} chy_extpot_desc;
+int chy_external_pot(chy_ctx *ctx, const chy_extpot_desc *d, const double *rhf_d, const double *zhf_d,
+                     double *phi_d, double *starphi_d);
+
 /* ---- stage timing (no reference counterpart; for benchmarks) ----
  * While enabled, every stage of chy_bdygen/chy_pot3/chy_solve is bracketed by
  * HIP events on the context's stream.  chy_ctx_stage_times() waits for the

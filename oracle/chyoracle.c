@@ -917,3 +917,36 @@ int ora_source_grav(int jmax, int kmax, int lmax, int pjmax, int pkmax, int jsta
 #undef PX
     return 0;
 }
+
+/*
+ * The central star's potential added to PHI: ExternalPot (pot/ExternalPot.f:1-88)
+ * with EXTERNAL_POT=1, INDIRECT=0, BINARY=0 (the default build's flags).  The
+ * star sits at (RPSTAR cos PHI_STAR, RPSTAR sin PHI_STAR) (ExternalPot.f:18-19);
+ * for J=1..JMAX+2, K=1..KMAX+2, L=1..LMAX (ExternalPot.f:43-51) STARPHI gets
+ * -(MASS_STAR+TMASSACC)/sqrt(RHF^2 + ZHF^2 + x*^2 + y*^2 - 2 RHF (cos psi x* +
+ * sin psi y*)) with psi = (L-1+1/2) DTHETA, and PHI += STARPHI.  Binary64,
+ * operations in the Fortran expression's order.  phi has the potential grid's
+ * leading dimensions (pjmax+2, pkmax+2), starphi the hydro grid's.
+ */
+int ora_external_pot(int jmax, int kmax, int lmax, int pjmax, int pkmax, double dtheta, double mass_star,
+                     double tmassacc, double rpstar, double phi_star, const double *rhf, const double *zhf,
+                     double *phi, double *starphi)
+{
+    if (jmax < 1 || kmax < 1 || lmax < 1 || pjmax < jmax || pkmax < kmax) return 1;
+    const long hj = jmax + 2, hk = kmax + 2, pj = pjmax + 2, pk = pkmax + 2;
+    const double ix = rpstar * cos(phi_star), iy = rpstar * sin(phi_star);
+    for (int L = 1; L <= lmax; ++L) {
+        const double psi = ((double)(L - 1) + 0.5) * dtheta;
+        const double c = cos(psi), s = sin(psi);
+        for (int K = 1; K <= kmax + 2; ++K)
+            for (int J = 1; J <= jmax + 2; ++J) {
+                const double rh = rhf[J - 1], zh = zhf[K - 1];
+                const double sp = -(mass_star + tmassacc) /
+                                  sqrt(rh * rh + zh * zh + ix * ix + iy * iy - 2.0 * rh * (c * ix + s * iy));
+                starphi[(L - 1) * hk * hj + (K - 1) * hj + (J - 1)] = sp;
+                const long q = (L - 1) * pk * pj + (K - 1) * pj + (J - 1);
+                phi[q] = phi[q] + sp;
+            }
+    }
+    return 0;
+}

@@ -43,6 +43,7 @@ def lib():
                                    C.c_double, d, d, d, d]
         L.ora_terms.argtypes = [i, i, d, i, d]
         L.ora_source_grav.argtypes = [C.c_int] * 6 + [C.c_double] * 4 + [d] * 8
+        L.ora_external_pot.argtypes = [C.c_int] * 5 + [C.c_double] * 5 + [d] * 4
         _libs[path] = L
     return _libs[path]
 
@@ -170,6 +171,22 @@ def source_grav(jmax, kmax, lmax, pjmax, pkmax, jstart, delt, dr, dz, dtheta, r,
     rc = lib().ora_source_grav(jmax, kmax, lmax, pjmax, pkmax, jstart, delt, dr, dz, dtheta, *[_p(x) for x in args])
     if rc:
         raise ValueError("ora_source_grav: bad arguments")
+
+
+def external_pot(jmax, kmax, lmax, pjmax, pkmax, dtheta, mass_star, tmassacc, rpstar, phi_star, rhf, zhf, phi,
+                 starphi):
+    """ExternalPot (pot/ExternalPot.f:1-88, INDIRECT=0, BINARY=0): starphi (Fortran
+    order, (JMAX+2, KMAX+2, LMAX)) is written and added to phi ((PJMAX+2,
+    PKMAX+2, LMAX)) in place."""
+    for x, shape in ((phi, (pjmax + 2, pkmax + 2, lmax)), (starphi, (jmax + 2, kmax + 2, lmax))):
+        if x.shape != shape or not x.flags.f_contiguous or x.dtype != np.float64:
+            raise ValueError(f"expected a Fortran-ordered float64 array of shape {shape}")
+    rhf = np.ascontiguousarray(rhf, dtype=np.float64)
+    zhf = np.ascontiguousarray(zhf, dtype=np.float64)
+    rc = lib().ora_external_pot(jmax, kmax, lmax, pjmax, pkmax, dtheta, mass_star, tmassacc, rpstar, phi_star,
+                                _p(rhf), _p(zhf), _p(phi), _p(starphi))
+    if rc:
+        raise ValueError("ora_external_pot: bad arguments")
 
 
 def set_threads(n: int):

@@ -8,7 +8,8 @@ difference identity, constant RHO and OMEGA an exact centrifugal term; every
 one uses dyadic values (exact arithmetic) or a stated tolerance, and the
 potential grid is larger than the hydro grid so that a pitch mix-up shows.
 GPU: chy_source_grav bitwise equal to the oracle, and the timestep loop (PHI
-zeroed, BDYGEN with REDGE=R(JMAX1) + POT3, then the sources) step by step.
+zeroed, BDYGEN with REDGE=R(JMAX1) + POT3, the star potential of ExternalPot,
+then the sources) step by step.
 """
 import numpy as np
 import pytest
@@ -202,11 +203,13 @@ def test_source_grav_rejects_bad_arguments(ctx):
 
 @pytest.mark.gpu
 def test_timestep_loop_with_device_resident_fields(ctx):
-    """Three steps of the hydro step's gravity part at 32x16x32: PHI zeroed
-    (3dhyd-main.f:457-465), BDYGEN(10,2,R(JMAX1)) + POT3(8) on the device
-    density, then the sources; the density changes between steps.  PHI is
-    checked against the oracle's solve at 1e-12, S/T/A bitwise against the
-    oracle's sources on the same PHI."""
+    """Three steps of the hydro step's gravity part at 32x16x32 in the
+    reference's order: PHI zeroed (3dhyd-main.f:457-465), BDYGEN(10,2,R(JMAX1))
+    + POT3(8) on the device density (:474-477), the star potential
+    (ExternalPot, :511), then the sources (SOURCE, :524); the density and the
+    star's position change between steps.  PHI is checked against the oracle's
+    solve at 1e-12; PHI after ExternalPot and S/T/A bitwise against the
+    oracle's ExternalPot and sources applied to the same arrays."""
     import chypot
     from chypot import fields
     from helpers import rel_max
@@ -222,6 +225,9 @@ def test_timestep_loop_with_device_resident_fields(ctx):
     phi_d = torch.empty_like(rho_d)
     r_d = torch.from_numpy(g.r()).to("cuda")
     d = chypot.SourceDesc(g.jmax, g.kmax, g.lmax, g.jmax, g.kmax, 2, 1.0e-3, g.delr, g.delz, g.dtheta)
+    rhf_d, zhf_d = torch.from_numpy(g.rhf()).to("cuda"), torch.from_numpy(g.zhf()).to("cuda")
+    sp_d = torch.empty_like(rho_d)
+    sp = np.zeros(g.shape, order="F")
     try:
         for step in range(3):
             print(f"[gpu] timestep {step}", flush=True)
@@ -231,6 +237,13 @@ def test_timestep_loop_with_device_resident_fields(ctx):
             want = po.solve(g, rho_h, maxtrm=10, isym=2, redge=redge, npoint=8)
             assert rel_max(phi, want["phi"]) < 1e-12
             assert abs(diag.tmass - want["tmass"]) <= 1e-13 * abs(want["tmass"])
+            e = chypot.ExtPotDesc(g.jmax, g.kmax, g.lmax, g.jmax, g.kmax, g.dtheta, 1.0, 1.0e-3 * step,
+                                  0.02, 0.4 * step)
+            ctx.external_pot(e, rhf_d, zhf_d, phi_d, sp_d)
+            po.external_pot(g.jmax, g.kmax, g.lmax, g.jmax, g.kmax, e.dtheta, e.mass_star, e.tmassacc, e.rpstar,
+                            e.phi_star, g.rhf(), g.zhf(), phi, sp)
+            assert np.array_equal(host(phi_d), phi), f"PHI after ExternalPot at step {step}"
+            assert np.array_equal(host(sp_d), sp), f"STARPHI at step {step}"
             ctx.source_grav(d, r_d, rho_d, om_d, phi_d, p_d, s_d, t_d, a_d)
             po.source_grav(g.jmax, g.kmax, g.lmax, g.jmax, g.kmax, 2, d.delt, d.dr, d.dz, d.dtheta, g.r(),
                            rho_h, omega, phi, p, s, t, a)
