@@ -24,8 +24,9 @@ The line also carries:
   * `source_grav`: the hydro step's momentum sources that read PHI
     (radhydro/source.f:41-79) on the same grid, per call, with the oracle's
     C loop timed beside it, and the star potential added to PHI before them
-    (pot/ExternalPot.f, `source_grav.external_pot`) (--source-calls 0 skips
-    both).
+    (pot/ExternalPot.f, `source_grav.external_pot`) and the disk's pull on
+    the star (find_force of pot/wiggle.F, `source_grav.find_force`)
+    (--source-calls 0 skips them).
 
     python bench.py [--steps K] [--warmup W] [--batch B] [--workers W] [--gpus 1]
 """
@@ -192,6 +193,19 @@ def source_row(ctx, stream, g, n, with_cpu):
     out["external_pot"] = {"kernel": "k_external_pot", "us_per_call": ms_e * 1e3, "algorithmic_bytes": eb,
                            "achieved_GBs": eb / (ms_e * 1e-3) / 1e9,
                            "frac_hbm": eb / (ms_e * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    # the disk's pull on the star (find_force of wiggle, JMIN=4): RHO read over
+    # J=1..JMAX+1, K=2..KMAX+1; each call reads (fx, fy) back and waits, so the
+    # time includes that round trip
+    fd = chypot.ForceDesc(g.jmax, g.kmax, g.lmax, 4, g.delr, g.delz, g.dtheta, 0.01)
+    ctx.find_force(fd, rhf, zhf, rho, 0.01, 0.0)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        ctx.find_force(fd, rhf, zhf, rho, 0.01, 0.0)
+    ms_f = (time.perf_counter() - t0) / n * 1e3
+    fb = 8 * (g.jmax + 1) * g.kmax * g.lmax
+    out["find_force"] = {"kernel": "k_find_force + k_force_total", "us_per_call": ms_f * 1e3,
+                         "algorithmic_bytes": fb, "achieved_GBs": fb / (ms_f * 1e-3) / 1e9,
+                         "includes": "launch, readback of (fx, fy) and the wait"}
     del rho, om, phi, p, s, t, a
     if with_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))

@@ -34,7 +34,7 @@ EXPORTS = (
     "chy_bdygen", "chy_pot3", "chy_solve", "chy_bdygen_host", "chy_pot3_host",
     "chy_ctx_profile", "chy_ctx_stage_times", "chy_ctx_graphs",
     "chy_zaxphi", "chy_zaxphi_host", "chy_phac", "chy_solve_batch", "chy_ctx_workers",
-    "chy_solve_refined", "chy_source_grav", "chy_external_pot",
+    "chy_solve_refined", "chy_source_grav", "chy_external_pot", "chy_find_force",
     "chy_strerror", "chy_last_device_error",
 )
 
@@ -66,6 +66,11 @@ class ExtPotDesc(C.Structure):
                 ("pot_jmax", C.c_int), ("pot_kmax", C.c_int), ("dtheta", C.c_double),
                 ("mass_star", C.c_double), ("tmassacc", C.c_double), ("rpstar", C.c_double),
                 ("phi_star", C.c_double)]
+
+
+class ForceDesc(C.Structure):
+    _fields_ = [("jmax", C.c_int), ("kmax", C.c_int), ("lmax", C.c_int), ("jmin", C.c_int),
+                ("dr", C.c_double), ("dz", C.c_double), ("dphi", C.c_double), ("gsoft", C.c_double)]
 
 
 class Diag(C.Structure):
@@ -116,6 +121,7 @@ def lib():
         L.chy_ctx_stage_times.argtypes = [vp, dp, C.POINTER(C.c_int), C.c_int]
         L.chy_source_grav.argtypes = [vp, C.POINTER(SourceDesc)] + [vp] * 8
         L.chy_external_pot.argtypes = [vp, C.POINTER(ExtPotDesc)] + [vp] * 4
+        L.chy_find_force.argtypes = [vp, C.POINTER(ForceDesc), vp, vp, vp, C.c_double, C.c_double, dp]
         L.chy_strerror.argtypes = [C.c_int]
         L.chy_strerror.restype = C.c_char_p
         L.chy_last_device_error.restype = C.c_char_p
@@ -213,6 +219,18 @@ class Context:
             raise ValueError("rhf/zhf: too few values")
         _check(lib().chy_external_pot(self.handle, C.byref(desc), _dptr(rhf), _dptr(zhf), _dptr(phi),
                                       _dptr(starphi)), "chy_external_pot")
+
+    def find_force(self, desc: ForceDesc, rhf, zhf, rho, x: float, y: float):
+        """find_force of pot/wiggle.F: the disk's pull (fx, fy) on a star at (x, y, 0);
+        rho is a device tensor (LMAX, KMAX+2, JMAX+2).  Waits."""
+        if tuple(rho.shape) != (desc.lmax, desc.kmax + 2, desc.jmax + 2):
+            raise ValueError(f"rho: shape {tuple(rho.shape)}")
+        if rhf.numel() < desc.jmax + 2 or zhf.numel() < desc.kmax + 2:
+            raise ValueError("rhf/zhf: too few values")
+        f = (C.c_double * 2)()
+        _check(lib().chy_find_force(self.handle, C.byref(desc), _dptr(rhf), _dptr(zhf), _dptr(rho), float(x),
+                                    float(y), f), "chy_find_force")
+        return float(f[0]), float(f[1])
 
     def stage_times(self):
         """{stage: (total_ms, launches)} since profiling was enabled (waits)."""

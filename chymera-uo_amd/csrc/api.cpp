@@ -471,6 +471,9 @@ int chy_ctx_destroy(chy_ctx *ctx)
     if (ctx->d_diag) (void)hipFree(ctx->d_diag);
     if (ctx->h_diag) (void)hipHostFree(ctx->h_diag);
     for (const chy_ctx::TrigTable &t : ctx->trig) (void)hipFree(t.d);
+    if (ctx->force_part) (void)hipFree(ctx->force_part);
+    if (ctx->force_out) (void)hipFree(ctx->force_out);
+    if (ctx->force_h) (void)hipHostFree(ctx->force_h);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
     return CHY_OK;
@@ -867,6 +870,77 @@ int chy_source_grav(chy_ctx *ctx, const chy_source_desc *d, const double *r_d, c
                   "SOURCE launch");
 }
 
+// (cos, sin) of the azimuth of plane L = 1..LMAX, cached per context.  kind 0:
+// (L-1+1/2) DTHETA as ExternalPot.f:46-48 forms it; kind 1: L DPHI - DPHI/2 as
+// wiggle.F:115 does.  Built on the host with the C library's cos/sin (the
+// first call for a key waits for the upload).
+static int trig_table(chy_ctx *ctx, int kind, int lmax, double dtheta, const double **out)
+{
+    for (const chy_ctx::TrigTable &t : ctx->trig)
+        if (t.kind == kind && t.lmax == lmax && t.dtheta == dtheta) {
+            *out = t.d;
+            return CHY_OK;
+        }
+    std::vector<double> h(2 * (size_t)lmax);
+    for (int l = 1; l <= lmax; ++l) {
+        const double a = kind == 0 ? ((double)(l - 1) + 0.5) * dtheta : (double)l * dtheta - 0.5 * dtheta;
+        h[2 * (l - 1)] = std::cos(a);
+        h[2 * (l - 1) + 1] = std::sin(a);
+    }
+    double *dev = nullptr;
+    if (hipMalloc(&dev, h.size() * sizeof(double)) != hipSuccess) return CHY_ENOMEM;
+    int rc = dev_rc((int)hipMemcpy(dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice),
+                    "cos/sin table upload");
+    if (rc) {
+        (void)hipFree(dev);
+        return rc;
+    }
+    ctx->trig.push_back(chy_ctx::TrigTable{kind, lmax, dtheta, dev});
+    *out = dev;
+    return CHY_OK;
+}
+
+int chy_find_force(chy_ctx *ctx, const chy_force_desc *d, const double *rhf_d, const double *zhf_d,
+                   const double *rho_d, double x, double y, double *f_h)
+{
+    if (!ctx) return CHY_ESTATE;
+    if (!d || !rhf_d || !zhf_d || !rho_d || !f_h) return CHY_EINVAL;
+    if (d->jmax < 1 || d->kmax < 1 || d->lmax < 1 || d->jmin < 4 || d->jmin > d->jmax + 1) return CHY_EINVAL;
+    if (!(d->dr > 0.0) || !(d->dz > 0.0) || !(d->dphi > 0.0)) return CHY_EINVAL;
+    DeviceScope ds(ctx->device);
+    const double *cs = nullptr;
+    int rc = trig_table(ctx, 1, d->lmax, d->dphi, &cs);
+    if (rc) return rc;
+    const int nb = force_blocks(d);
+    if (nb > ctx->force_cap) {
+        if (ctx->force_part) (void)hipFree(ctx->force_part);
+        ctx->force_part = nullptr;
+        ctx->force_cap = 0;
+        if (hipMalloc(&ctx->force_part, sizeof(double) * 2 * (size_t)nb) != hipSuccess) return CHY_ENOMEM;
+        ctx->force_cap = nb;
+    }
+    if (!ctx->force_out) {
+        if (hipMalloc(&ctx->force_out, sizeof(double) * 2) != hipSuccess) return CHY_ENOMEM;
+        if (hipHostMalloc(&ctx->force_h, sizeof(double) * 2) != hipSuccess) {
+            (void)hipFree(ctx->force_out);
+            ctx->force_out = nullptr;
+            return CHY_ENOMEM;
+        }
+    }
+    hipStream_t s = ctx->stream;
+    rc = dev_rc(launch_find_force(d, rhf_d, zhf_d, cs, rho_d, x, y, ctx->force_part, ctx->force_out, s),
+                "find_force launch");
+    if (!rc)
+        rc = dev_rc((int)hipMemcpyAsync(ctx->force_h, ctx->force_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s),
+                    "force readback");
+    if (!rc) rc = dev_rc((int)hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (!rc) {
+        f_h[0] = ctx->force_h[0];
+        f_h[1] = ctx->force_h[1];
+    }
+    return rc;
+}
+
 int chy_external_pot(chy_ctx *ctx, const chy_extpot_desc *d, const double *rhf_d, const double *zhf_d,
                      double *phi_d, double *starphi_d)
 {
@@ -877,27 +951,8 @@ int chy_external_pot(chy_ctx *ctx, const chy_extpot_desc *d, const double *rhf_d
     if (!(d->dtheta > 0.0)) return CHY_EINVAL;
     DeviceScope ds(ctx->device);
     const double *cs = nullptr;
-    for (const chy_ctx::TrigTable &t : ctx->trig)
-        if (t.lmax == d->lmax && t.dtheta == d->dtheta) cs = t.d;
-    if (!cs) {
-        // psi_L = (L-1+1/2) DTHETA and its cos/sin as ExternalPot.f:46-48 forms them
-        std::vector<double> h(2 * (size_t)d->lmax);
-        for (int l = 1; l <= d->lmax; ++l) {
-            const double psi = ((double)(l - 1) + 0.5) * d->dtheta;
-            h[2 * (l - 1)] = std::cos(psi);
-            h[2 * (l - 1) + 1] = std::sin(psi);
-        }
-        double *dev = nullptr;
-        if (hipMalloc(&dev, h.size() * sizeof(double)) != hipSuccess) return CHY_ENOMEM;
-        int rc = dev_rc((int)hipMemcpy(dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice),
-                        "cos/sin table upload");
-        if (rc) {
-            (void)hipFree(dev);
-            return rc;
-        }
-        ctx->trig.push_back(chy_ctx::TrigTable{d->lmax, d->dtheta, dev});
-        cs = dev;
-    }
+    int rc = trig_table(ctx, 0, d->lmax, d->dtheta, &cs);
+    if (rc) return rc;
     // the star's position as ExternalPot.f:18-19 forms it
     const double ix = d->rpstar * std::cos(d->phi_star), iy = d->rpstar * std::sin(d->phi_star);
     return dev_rc(launch_external_pot(d, rhf_d, zhf_d, cs, phi_d, starphi_d, ix, iy, ctx->stream),

@@ -128,11 +128,15 @@ struct chy_ctx {
     int diag_cap = 0;
     // chy_external_pot: (cos psi_L, sin psi_L) tables per (LMAX, DTHETA)
     struct TrigTable {
+        int kind;  // 0: psi_L = (L-1+1/2) DTHETA (ExternalPot); 1: L DPHI - DPHI/2 (find_force)
         int lmax;
         double dtheta;
         double *d;
     };
     std::vector<TrigTable> trig;
+    // chy_find_force: block partials and the (fx, fy) result
+    double *force_part = nullptr, *force_out = nullptr, *force_h = nullptr;
+    int force_cap = 0;
 };
 
 namespace chy {
@@ -261,6 +265,11 @@ bool bdygen_spectral_ok(const chy_grid *g);
 // holds (cos psi_L, sin psi_L) for L = 1..LMAX, ix/iy the star's position).
 int launch_external_pot(const chy_extpot_desc *d, const double *rhf, const double *zhf, const double *cs,
                         double *phi, double *starphi, double ix, double iy, hipStream_t st);
+// find_force of pot/wiggle.F: block partials into part (force_blocks(d) pairs),
+// the total (fx, fy) into out.
+int force_blocks(const chy_force_desc *d);
+int launch_find_force(const chy_force_desc *d, const double *rhf, const double *zhf, const double *cs,
+                      const double *rho, double x, double y, double *part, double *out, hipStream_t st);
 // The momentum sources of radhydro/source.f:41-79 (arguments checked by the caller).
 int launch_source_grav(const chy_source_desc *d, const double *r, const double *rho, const double *omega,
                        const double *phi, const double *p, double *s, double *t, double *a, hipStream_t st);

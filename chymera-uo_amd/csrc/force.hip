@@ -1,0 +1,131 @@
+// The disk's gravitational pull on the central star: find_force
+// (pot/wiggle.F:99-135), which wiggle (pot/wiggle.F:15-93) calls every hydro
+// step (radhydro/3dhyd-main.f:479-486) to advance the star by leapfrog.
+// Every cell is a point mass RHO*RHF*VOL at its centre (VOL = DR*DZ*DPHI*2,
+// both halves of the disk) and pulls the star at (x, y, 0) with softening
+// GSOFT:
+//
+//   (fx, fy) = sum over L=1..LMAX, K=2..KMAX+1, J=JMIN-3..JMAX+1 of
+//              RHO*RHF*VOL * (xx, yy) / sqrt(xx^2 + yy^2 + ZHF^2 + GSOFT^2)^3,
+//   xx = RHF cos(phi_L) - x,  yy = RHF sin(phi_L) - y,  phi_L = L DPHI - DPHI/2.
+//
+// The reference sums in OpenMP reduction order (it depends on the thread
+// count); here each thread sums its cells over a run of FORCE_LC planes, the
+// block reduces its threads in a fixed tree, and one block adds the block
+// partials in block order -- a fixed order, so results repeat run to run.
+//
+// RHO is read with the leading dimensions (JMAX+2, KMAX+2) that find_force
+// declares (pot/wiggle.F:103), as the reference reads the array it is passed.
+// One thread per (J, K) and run of planes; a wavefront reads 64 consecutive J
+// of one row per plane.  cos/sin of phi_L come from a table.
+#include "chypot_impl.h"
+
+namespace chy {
+namespace {
+
+struct ForceArgs {
+    const double *rhf, *zhf, *cs, *rho;  // cs: (cos phi_L, sin phi_L) pairs
+    double *part;                        // 2 doubles per block
+    int j2, k2, j0, nj, kmax, lmax;      // RHO pitches JMAX+2, KMAX+2; first J (JMIN-3) and count
+    double x, y, vol, gs2;
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+constexpr int FORCE_LC = 16;  // planes L per thread
+
+__global__ __launch_bounds__(256) void k_find_force(ForceArgs a)
+{
+#pragma clang fp contract(off)
+    const int t = (int)threadIdx.x;
+    const int J = a.j0 + (int)blockIdx.x * 64 + (t & 63);
+    const int K = 2 + (int)blockIdx.y * 4 + (t >> 6);
+    const int L0 = (int)blockIdx.z * FORCE_LC, L1 = min(L0 + FORCE_LC, a.lmax);
+    double fx = 0.0, fy = 0.0;
+    if (J < a.j0 + a.nj && K <= a.kmax + 1) {
+        // each cell's terms in the order of wiggle.F:119-126
+        const double rh = a.rhf[J - 1], zz = a.zhf[K - 1];
+        const double z2 = zz * zz;
+        const size_t plane = (size_t)a.j2 * a.k2;
+        const double *rp = a.rho + (size_t)(J - 1) + (size_t)a.j2 * (K - 1);
+        for (int L = L0; L < L1; ++L) {
+            const double xx = rh * a.cs[2 * L] - a.x, yy = rh * a.cs[2 * L + 1] - a.y;
+            const double s = sqrt(xx * xx + yy * yy + z2 + a.gs2);
+            const double dd = 1.0 / (s * s * s);
+            const double tm = rp[plane * L] * rh * a.vol;
+            fx += tm * dd * xx;
+            fy += tm * dd * yy;
+        }
+    }
+    __shared__ double red[2][4];
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = fx;
+        red[1][t >> 6] = fy;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        a.part[2 * b] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        a.part[2 * b + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    }
+}
+
+// One block adds the partials: each thread a strided share in order, then a
+// fixed tree.
+__global__ __launch_bounds__(256) void k_force_total(const double *part, int nparts, double *out)
+{
+    const int t = (int)threadIdx.x;
+    double fx = 0.0, fy = 0.0;
+    for (int i = t; i < nparts; i += 256) {
+        fx += part[2 * i];
+        fy += part[2 * i + 1];
+    }
+    __shared__ double red[2][4];
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = fx;
+        red[1][t >> 6] = fy;
+    }
+    __syncthreads();
+    if (t == 0) {
+        out[0] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        out[1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    }
+}
+
+}  // namespace
+
+int force_blocks(const chy_force_desc *d)
+{
+    const int nj = d->jmax + 1 - (d->jmin - 3) + 1;
+    return ((nj + 63) / 64) * ((d->kmax + 3) / 4) * ((d->lmax + FORCE_LC - 1) / FORCE_LC);
+}
+
+int launch_find_force(const chy_force_desc *d, const double *rhf, const double *zhf, const double *cs,
+                      const double *rho, double x, double y, double *part, double *out, hipStream_t st)
+{
+    ForceArgs a{};
+    a.rhf = rhf; a.zhf = zhf; a.cs = cs; a.rho = rho; a.part = part;
+    a.j2 = d->jmax + 2; a.k2 = d->kmax + 2;  // the leading dimensions wiggle.F:103 declares
+    a.j0 = d->jmin - 3; a.nj = d->jmax + 1 - a.j0 + 1;
+    a.kmax = d->kmax; a.lmax = d->lmax;
+    a.x = x; a.y = y;
+    a.vol = d->dr * d->dz * d->dphi * 2.0;
+    a.gs2 = d->gsoft * d->gsoft;
+    const dim3 grid((a.nj + 63) / 64, (d->kmax + 3) / 4, (d->lmax + FORCE_LC - 1) / FORCE_LC);
+    hipLaunchKernelGGL(k_find_force, grid, dim3(256), 0, st, a);
+    int rc = (int)hipGetLastError();
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_force_total, dim3(1), dim3(256), 0, st, part, (int)(grid.x * grid.y * grid.z), out);
+    return (int)hipGetLastError();
+}
+
+}  // namespace chy

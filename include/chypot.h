@@ -241,6 +241,25 @@ typedef struct {
 int chy_external_pot(chy_ctx *ctx, const chy_extpot_desc *d, const double *rhf_d, const double *zhf_d,
                      double *phi_d, double *starphi_d);
 
+/* ---- the disk's pull on the star (pot/wiggle.F:99-135) ----
+ * find_force, which wiggle calls every hydro step to move the star: the sum
+ * over L=1..LMAX, K=2..KMAX+1, J=JMIN-3..JMAX+1 of the point-mass force of
+ * each cell (mass RHO*RHF*DR*DZ*DPHI*2 at (RHF, phi_L = L DPHI - DPHI/2,
+ * ZHF), softening GSOFT) on a star at (x, y, 0).  rho_d is read with the
+ * leading dimensions (JMAX+2, KMAX+2) find_force declares (wiggle.F:103);
+ * rhf_d, zhf_d as for chy_external_pot.  The sum runs in a fixed order (one
+ * partial per thread block, added in block order), so results repeat run to
+ * run; the reference's OpenMP order differs, so they agree with it to
+ * rounding.  Writes (fx, fy) to f_h (host) and waits.  CHY_EINVAL for null
+ * pointers, JMIN outside 4..JMAX+1 (find_force reads J=JMIN-3), or
+ * spacings not > 0. */
+typedef struct {
+    int jmax, kmax, lmax, jmin;   /* JMAX, KMAX, LMAX, JMIN */
+    double dr, dz, dphi, gsoft;   /* ROF3N, ZOF3N, DTHETA, GSOFT */
+} chy_force_desc;
+int chy_find_force(chy_ctx *ctx, const chy_force_desc *d, const double *rhf_d, const double *zhf_d,
+                   const double *rho_d, double x, double y, double *f_h);
+
 /* ---- stage timing (no reference counterpart; for benchmarks) ----
  * While enabled, every stage of chy_bdygen/chy_pot3/chy_solve is bracketed by
  * HIP events on the context's stream.  chy_ctx_stage_times() waits for the

@@ -950,3 +950,48 @@ int ora_external_pot(int jmax, int kmax, int lmax, int pjmax, int pkmax, double 
     }
     return 0;
 }
+
+/*
+ * The disk's pull on the star: find_force (pot/wiggle.F:99-135), summed in the
+ * loop order of one OpenMP thread (L, then K, then J).  rho has the leading
+ * dimensions (JMAX+2, KMAX+2) find_force declares (wiggle.F:103).  Each term
+ * is formed in binary64 as the reference forms it; the sums are kept in the
+ * working precision (binary64, or 80-bit in the extended build, where they are
+ * close to the exact sums of the binary64 terms).  f gets (fx, fy); fabs_sum
+ * (may be NULL) gets the sums of |terms|, the scale of the rounding
+ * differences between summation orders.
+ */
+int ora_find_force(int jmax, int kmax, int lmax, int jmin, double dr, double dz, double dphi, double gsoft,
+                   const double *rhf, const double *zhf, const double *rho, double x, double y, double *f,
+                   double *fabs_sum)
+{
+    if (jmax < 1 || kmax < 1 || lmax < 1 || jmin < 4 || jmin > jmax + 1) return 1;
+    const long hj = jmax + 2, hk = kmax + 2;
+    const double vol = dr * dz * dphi * 2.0;                      /* wiggle.F:107 */
+    real fx = 0, fy = 0, ax = 0, ay = 0;
+    for (int L = 1; L <= lmax; ++L) {
+        const double lphi = (double)L * dphi - 0.5 * dphi;         /* wiggle.F:115 */
+        for (int K = 2; K <= kmax + 1; ++K) {
+            const double zz = zhf[K - 1];
+            for (int J = jmin - 3; J <= jmax + 1; ++J) {
+                const double xx = rhf[J - 1] * cos(lphi) - x;
+                const double yy = rhf[J - 1] * sin(lphi) - y;
+                const double tmass = rho[(L - 1) * hk * hj + (K - 1) * hj + (J - 1)] * rhf[J - 1] * vol;
+                const double s = sqrt(xx * xx + yy * yy + zz * zz + gsoft * gsoft);
+                const double dd = 1.0 / (s * s * s);                /* wiggle.F:123 */
+                const double tx = tmass * dd * xx, ty = tmass * dd * yy;
+                fx = fx + tx;
+                fy = fy + ty;
+                ax += fabs(tx);
+                ay += fabs(ty);
+            }
+        }
+    }
+    f[0] = (double)fx;
+    f[1] = (double)fy;
+    if (fabs_sum) {
+        fabs_sum[0] = (double)ax;
+        fabs_sum[1] = (double)ay;
+    }
+    return 0;
+}

@@ -44,6 +44,7 @@ def lib():
         L.ora_terms.argtypes = [i, i, d, i, d]
         L.ora_source_grav.argtypes = [C.c_int] * 6 + [C.c_double] * 4 + [d] * 8
         L.ora_external_pot.argtypes = [C.c_int] * 5 + [C.c_double] * 5 + [d] * 4
+        L.ora_find_force.argtypes = [C.c_int] * 4 + [C.c_double] * 4 + [d] * 3 + [C.c_double] * 2 + [d] * 2
         _libs[path] = L
     return _libs[path]
 
@@ -187,6 +188,22 @@ def external_pot(jmax, kmax, lmax, pjmax, pkmax, dtheta, mass_star, tmassacc, rp
                                 _p(rhf), _p(zhf), _p(phi), _p(starphi))
     if rc:
         raise ValueError("ora_external_pot: bad arguments")
+
+
+def find_force(jmax, kmax, lmax, jmin, dr, dz, dphi, gsoft, rhf, zhf, rho, x, y):
+    """find_force (pot/wiggle.F:99-135) in one thread's loop order (sums in
+    80-bit arithmetic under use_extended(True)): returns
+    ((fx, fy), (sum |x terms|, sum |y terms|)); rho Fortran-ordered (JMAX+2,
+    KMAX+2, LMAX)."""
+    if rho.shape != (jmax + 2, kmax + 2, lmax) or not rho.flags.f_contiguous or rho.dtype != np.float64:
+        raise ValueError("rho: expected a Fortran-ordered float64 array of shape (JMAX+2, KMAX+2, LMAX)")
+    rhf = np.ascontiguousarray(rhf, dtype=np.float64)
+    zhf = np.ascontiguousarray(zhf, dtype=np.float64)
+    f, a = np.zeros(2), np.zeros(2)
+    if lib().ora_find_force(jmax, kmax, lmax, jmin, dr, dz, dphi, gsoft, _p(rhf), _p(zhf), _p(rho), x, y, _p(f),
+                            _p(a)):
+        raise ValueError("ora_find_force: bad arguments")
+    return (float(f[0]), float(f[1])), (float(a[0]), float(a[1]))
 
 
 def set_threads(n: int):

@@ -201,15 +201,31 @@ def test_source_grav_rejects_bad_arguments(ctx):
         assert ei.value.code == chypot.CHY_EINVAL
 
 
+def wiggle_step(star, x, y, f, delt):
+    """The star's leapfrog step of wiggle (pot/wiggle.F:55-90) with the pull f."""
+    star["vx"] += f[0] * star["odelt"] * 0.5
+    star["vy"] += f[1] * star["odelt"] * 0.5
+    star["vx"] += f[0] * delt * 0.5
+    star["vy"] += f[1] * delt * 0.5
+    cx, cy = x + star["vx"] * delt, y + star["vy"] * delt
+    star["odelt"] = delt
+    star["r"] = float(np.sqrt(cx * cx + cy * cy))
+    ph = float(np.arctan2(cy, cx))
+    star["phi"] = ph + 2 * np.pi if ph < 0 else ph
+
+
 @pytest.mark.gpu
 def test_timestep_loop_with_device_resident_fields(ctx):
     """Three steps of the hydro step's gravity part at 32x16x32 in the
     reference's order: PHI zeroed (3dhyd-main.f:457-465), BDYGEN(10,2,R(JMAX1))
-    + POT3(8) on the device density (:474-477), the star potential
-    (ExternalPot, :511), then the sources (SOURCE, :524); the density and the
-    star's position change between steps.  PHI is checked against the oracle's
-    solve at 1e-12; PHI after ExternalPot and S/T/A bitwise against the
-    oracle's ExternalPot and sources applied to the same arrays."""
+    + POT3(8) on the device density (:474-477), the star moved by wiggle with
+    the disk's pull from find_force (:479-486; the leapfrog itself is scalar
+    host arithmetic, done here as wiggle.F:55-90 does it), the star potential
+    (ExternalPot, :511), then the sources (SOURCE, :524); the density changes
+    between steps.  PHI is checked against the oracle's solve at 1e-12, the
+    pull against the oracle's to 1e-13 of the sum of |terms|, PHI after
+    ExternalPot and S/T/A bitwise against the oracle's ExternalPot and sources
+    applied to the same arrays."""
     import chypot
     from chypot import fields
     from helpers import rel_max
@@ -228,6 +244,8 @@ def test_timestep_loop_with_device_resident_fields(ctx):
     rhf_d, zhf_d = torch.from_numpy(g.rhf()).to("cuda"), torch.from_numpy(g.zhf()).to("cuda")
     sp_d = torch.empty_like(rho_d)
     sp = np.zeros(g.shape, order="F")
+    fd = chypot.ForceDesc(g.jmax, g.kmax, g.lmax, 4, g.delr, g.delz, g.dtheta, 0.01)
+    star = dict(r=0.02, phi=0.4, vx=0.0, vy=0.0, odelt=d.delt)  # wiggle's state
     try:
         for step in range(3):
             print(f"[gpu] timestep {step}", flush=True)
@@ -237,8 +255,14 @@ def test_timestep_loop_with_device_resident_fields(ctx):
             want = po.solve(g, rho_h, maxtrm=10, isym=2, redge=redge, npoint=8)
             assert rel_max(phi, want["phi"]) < 1e-12
             assert abs(diag.tmass - want["tmass"]) <= 1e-13 * abs(want["tmass"])
+            x, y = star["r"] * np.cos(star["phi"]), star["r"] * np.sin(star["phi"])
+            f = ctx.find_force(fd, rhf_d, zhf_d, rho_d, x, y)
+            want_f, scale = po.find_force(g.jmax, g.kmax, g.lmax, 4, g.delr, g.delz, g.dtheta, 0.01, g.rhf(),
+                                          g.zhf(), rho_h, x, y)
+            assert all(abs(f[i] - want_f[i]) <= 1e-13 * scale[i] for i in range(2)), (f, want_f)
+            wiggle_step(star, x, y, f, d.delt)
             e = chypot.ExtPotDesc(g.jmax, g.kmax, g.lmax, g.jmax, g.kmax, g.dtheta, 1.0, 1.0e-3 * step,
-                                  0.02, 0.4 * step)
+                                  star["r"], star["phi"])
             ctx.external_pot(e, rhf_d, zhf_d, phi_d, sp_d)
             po.external_pot(g.jmax, g.kmax, g.lmax, g.jmax, g.kmax, e.dtheta, e.mass_star, e.tmassacc, e.rpstar,
                             e.phi_star, g.rhf(), g.zhf(), phi, sp)
