@@ -10,6 +10,10 @@ C                        it (header line, then DENNY(HJ2,HK2) list-directed;
 C                        RHO(J,K,L)=DENNY(J,K) floored at GRIDLIM*DENCEN)
 C             | 'rho3d'  stream file RHOFILE: real*8 DELR, DELZ, then
 C                        RHO(JMAX2,KMAX2,LMAX) (3-D fields)
+C             | 'restart' restart model RHOFILE (fort.7 / saved.NNNNNN) read
+C                        as io.f:464-475 reads it: S, T, A, RHO, EPS, then
+C                        ROF3N..OMMAX; RHO and the spacings are used and
+C                        TIME replaces the namelist's
 C     RHOFILE = 'rho3d.bin'
 C     ISYM=2, MAXTRM=10, NPOINT=8, IPRINT=1, REDGE=0  (io.f:208-218 and
 C                        radhydro/3dhyd-main.f:166-167)
@@ -87,8 +91,19 @@ C io.f:136-140.
         CLOSE(2)
         ROF3N=DELR
         ZOF3N=DELZ
+      ELSE IF (TRIM(INPUT).EQ.'restart') THEN
+        OPEN(UNIT=2,FILE=TRIM(RHOFILE),FORM='UNFORMATTED',STATUS='OLD')
+        READ(2)
+        READ(2)
+        READ(2)
+        READ(2) RHO
+        READ(2)
+        READ(2) ROF3N,ZOF3N,DELT,TIME,ELOST,DEN,SOUND,JREQ,OMMAX
+        CLOSE(2)
+        DELR=ROF3N
+        DELZ=ZOF3N
       ELSE
-        WRITE(0,*) 'potdiag: INPUT must be fort2 or rho3d'
+        WRITE(0,*) 'potdiag: INPUT must be fort2, rho3d or restart'
         STOP 2
       ENDIF
 
