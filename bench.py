@@ -12,14 +12,19 @@ HBM.  B=1 is the single-field solve.
 
 The line also carries:
   * p50/p99 latency of one solve (a separate pass of single-field chy_solve
-    calls timed with HIP events on the solve's stream) and of one batch step;
+    calls timed with HIP events on the solve's stream; --latency-steps 0 skips
+    it) and of one batch step;
   * the error of field 0 against the fixture solutions (reference FP64 and the
     80-bit solve of the same discrete system);
-  * per-stage times and the roofline of the dominant kernel, from a pass of
-    single-field solves launched kernel by kernel with HIP events around every
-    stage on the solve's stream; `roofline.traffic` is the HBM bytes per launch
-    that rocprofv3 FETCH_SIZE/WRITE_SIZE recorded for that kernel
-    (profiles/potsolve_256x128x256_hbm_traffic.txt), or null;
+  * the roofline of the dominant kernel of the timed region: HIP events
+    around every stage on the worker stream the batched launches run on give
+    its average launch duration (`batch_stages_ms_per_launch`), one launch
+    processes `fields_per_launch` fields; `roofline.traffic` is the HBM bytes
+    per launch that rocprofv3 FETCH_SIZE/WRITE_SIZE recorded for that kernel
+    with launches of that many fields (profiles/potsolve_256x128x256_batch16_
+    hbm_traffic.txt, or ..._hbm_traffic.txt for single fields), or null;
+  * `stages_ms`: per-stage times of single-field solves launched kernel by
+    kernel (--stage-steps 0 skips them);
   * the reference CPU solve timed on this host;
   * `source_grav`: the hydro step's momentum sources that read PHI
     (radhydro/source.f:41-79) on the same grid, per call, with the oracle's
@@ -76,11 +81,14 @@ STAGE_KERNEL = {"zfwd": "k_zgemm_reg<true, 128, 64, 1>", "zinv": "k_zgemm_reg<fa
                 "rsolve": "k_rsolve<4>"}
 
 
-def hbm_traffic(stage):
+def hbm_traffic(stage, batch):
     """HBM bytes per launch of the stage's kernel as rocprofv3 FETCH_SIZE +
     WRITE_SIZE recorded them, each scaled by what the counter read for a
-    device-to-device copy of known size (profiles/..._hbm_traffic.txt)."""
-    f = ROOT / "profiles" / "potsolve_256x128x256_hbm_traffic.txt"
+    device-to-device copy of known size: profiles/..._hbm_traffic.txt for
+    single-field launches, profiles/..._batch<B>_hbm_traffic.txt for launches
+    of B fields."""
+    name_b = "" if batch == 1 else f"_batch{batch}"
+    f = ROOT / "profiles" / f"potsolve_256x128x256{name_b}_hbm_traffic.txt"
     name = STAGE_KERNEL.get(stage)
     if name is None or not f.exists():
         return None
@@ -279,6 +287,10 @@ def main():
                          "(60%% 128x64x128, 30%% 256x128x256, 10%% 512x256x512)")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="arrivals per second")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="HBM budget of the server")
+    ap.add_argument("--latency-steps", type=int, default=-1,
+                    help="single-field chy_solve calls for p50/p99 (default: --steps; 0 skips)")
+    ap.add_argument("--stage-steps", type=int, default=-1,
+                    help="single-field stage-timed solves for stages_ms (default: --steps; 0 skips)")
     ap.add_argument("--source-calls", type=int, default=20,
                     help="also time this many calls of the SOURCE momentum sources (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -330,9 +342,14 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    # single-field passes (latency, stage times) run only when asked for, so
+    # that a profile of the batch alone sees only batched launches
+    n_lat = args.steps if args.latency_steps < 0 else args.latency_steps
+    n_st = args.steps if args.stage_steps < 0 else args.stage_steps
     for _ in range(args.warmup):
         step()
-        single()
+        if n_lat > 0 or n_st > 0:
+            single()
     torch.cuda.synchronize()
 
     def timed(fn, n):
@@ -348,12 +365,17 @@ def main():
         t1 = time.perf_counter()
         return t1 - t0, np.array([a.elapsed_time(b) for a, b in zip(starts, ends)])
 
-    # timed region: batch steps (graph replays on the worker streams)
+    # timed region: batch steps (batched launches on a worker stream), with
+    # HIP events around every stage on the stream each launch runs on: the
+    # per-launch durations of the roofline come from the timed region itself
+    ctx.profile(True)
     elapsed, lat_batch = timed(step, args.steps)
+    tstages = ctx.stage_times()
+    ctx.profile(False)
     value = B * args.steps / elapsed
     phi = chypot.to_host(j0["phi"])
     # single-solve latency: one field per call on the context's stream
-    _, lat = timed(single, args.steps)
+    lat = timed(single, n_lat)[1] if n_lat > 0 else None
 
     # the refined path (binary32 correction solves on a binary64 residual):
     # single-field solves of field 0, timed the same way, and its error
@@ -375,12 +397,14 @@ def main():
         phi_ref = chypot.to_host(ref_phi)
 
     # single-field steps launched kernel by kernel with HIP events around every
-    # stage on the solve's stream: per-kernel durations for the roofline
-    ctx.profile(True)
-    for _ in range(args.steps):
-        single()
-    stages = ctx.stage_times()
-    ctx.profile(False)
+    # stage on the solve's stream: the single-field stage times
+    stages = None
+    if n_st > 0:
+        ctx.profile(True)
+        for _ in range(n_st):
+            single()
+        stages = ctx.stage_times()
+        ctx.profile(False)
 
     # accuracy of the timed result against the fixture solutions
     fx = np.load(ROOT / "tests" / "golden" / "solve_large_256x128x256.npz")
@@ -402,23 +426,25 @@ def main():
     }
     e_ext = max(err(phi[:, 1, :], fx["ext_phi_k2"]), err(phi[:, :, 0], fx["ext_phi_l1"]))
 
+    # roofline of the dominant stage of the timed region: its average launch
+    # duration and the fields one launch processes (B when the batch is one
+    # group of batched launches)
     work = stage_work(g)
-    stage_avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in stages.items()}
-    dom = max(stage_avg, key=stage_avg.get)
-    avg_ms = stage_avg[dom]
+    t_avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in tstages.items()}
+    dom = max(t_avg, key=lambda k: tstages[k][0])
+    avg_ms, launches = t_avg[dom], tstages[dom][1]
+    per_launch = B * args.steps / launches
     wk = work[dom]
     if wk["flops"] is not None and wk["flops"] / wk["bytes"] > FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9):
-        achieved = wk["flops"] / (avg_ms * 1e-3) / 1e12
+        achieved = per_launch * wk["flops"] / (avg_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": dom,
-                "avg_ms": avg_ms, "launches": stages[dom][1]}
+                "frac": achieved / FP64_PEAK_TFLOPS}
     else:
-        achieved = (wk["bytes"] or 0) / (avg_ms * 1e-3) / 1e9
+        achieved = per_launch * (wk["bytes"] or 0) / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "avg_ms": avg_ms,
-                "launches": stages[dom][1]}
-
-    roof["traffic"] = hbm_traffic(dom)
+                "frac": achieved / HBM_PEAK_GBS}
+    roof.update({"traffic": hbm_traffic(dom, int(round(per_launch))), "kernel": dom, "avg_ms": avg_ms,
+                 "launches": launches, "fields_per_launch": per_launch})
     hetero = hetero_batch(ctx, args.hetero) if args.hetero > 0 else None
     serving = serve_run(ctx, args) if args.serve > 0 else None
     source = (source_row(ctx, stream, g, args.source_calls, not args.no_cpu_baseline)
@@ -440,9 +466,9 @@ def main():
         "config": {"workload": "plain FP64 potential solve (BDYGEN+POT3), spiral-arm torus",
                    "grid": g.name, "path": "plain_fp64", "batch": B, "workers": W,
                    "coef": "K=2 plane", "maxtrm": 10, "isym": 2, "npoint": 8},
-        "p50_ms": float(np.percentile(lat, 50)),
-        "p99_ms": float(np.percentile(lat, 99)),
-        "single_solves_per_s": float(1e3 / np.mean(lat)),
+        "p50_ms": None if lat is None else float(np.percentile(lat, 50)),
+        "p99_ms": None if lat is None else float(np.percentile(lat, 99)),
+        "single_solves_per_s": None if lat is None else float(1e3 / np.mean(lat)),
         "batch_step_p50_ms": float(np.percentile(lat_batch, 50)),
         "batch_step_p99_ms": float(np.percentile(lat_batch, 99)),
         "max_rel_err_vs_reference_fp64": e_ref,
@@ -451,7 +477,9 @@ def main():
         "hetero_batch": hetero,
         "serving": serving,
         "source_grav": source,
-        "stages_ms": {k: round(v, 5) for k, v in stage_avg.items()},
+        "stages_ms": None if stages is None else {k: round(v[0] / v[1] if v[1] else 0.0, 5)
+                                                   for k, v in stages.items()},
+        "batch_stages_ms_per_launch": {k: round(v, 5) for k, v in t_avg.items() if tstages[k][1]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
