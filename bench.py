@@ -76,9 +76,9 @@ def stage_work(g):
 
 
 # stage -> kernel whose rocprofv3 counters stand for it
-STAGE_KERNEL = {"zfwd": "k_zgemm_reg<true, 128, 64, 1>", "zinv": "k_zgemm_reg<false, 128, 64, 1>",
-                "fft_fwd": "k_fft_forward<16, 8, 32>", "fft_inv": "k_fft_inverse<16, 8, 32>",
-                "rsolve": "k_rsolve<4>"}
+STAGE_KERNEL = {"zfwd": "k_zgemm_reg<true, 128, 64, 1, double>", "zinv": "k_zgemm_reg<false, 128, 64, 1, double>",
+                "fft_fwd": "k_fft_forward<16, 8, 32, double, double>",
+                "fft_inv": "k_fft_inverse<16, 8, 32, double, false>", "rsolve": "k_rsolve<4, double>"}
 
 
 def hbm_traffic(stage, batch):
@@ -93,9 +93,10 @@ def hbm_traffic(stage, batch):
     if name is None or not f.exists():
         return None
     for line in f.read_text().splitlines():
-        if line.startswith("kernel=") and name in line:
+        if line.startswith("kernel="):
             kv = dict(p.split("=", 1) for p in line.split("\t"))
-            return float(kv["traffic_bytes"])
+            if kv["kernel"] == name:
+                return float(kv["traffic_bytes"])
     return None
 
 

@@ -102,11 +102,38 @@ static int check_pot3_args(const chy_grid *g, int &npoint, int coef_mode, const 
 }
 
 // ---- stage profiler ----
+hipEvent_t Profiler::take()
+{
+    hipEvent_t e = nullptr;
+    if (!pool.empty()) {
+        e = pool.back();
+        pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+    }
+    return e;
+}
+
+void Profiler::reserve(size_t n)
+{
+    while (pool.size() < n) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        pool.push_back(e);
+    }
+}
+
+void Profiler::release()
+{
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+}
+
 void Profiler::begin(int stage, hipStream_t s)
 {
     if (!on) return;
-    hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return;
+    hipEvent_t e = take();
+    if (!e) return;
     (void)hipEventRecord(e, s);
     open_stage = stage;
     open_ev = e;
@@ -115,8 +142,8 @@ void Profiler::begin(int stage, hipStream_t s)
 void Profiler::end(hipStream_t s)
 {
     if (!on || open_stage < 0) return;
-    hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return;
+    hipEvent_t e = take();
+    if (!e) return;
     (void)hipEventRecord(e, s);
     pending.push_back(Rec{open_stage, open_ev, e});
     open_stage = -1;
@@ -136,8 +163,8 @@ int Profiler::collect()
         } else {
             rc = dev_rc((int)e, "stage timing");
         }
-        (void)hipEventDestroy(r.a);
-        (void)hipEventDestroy(r.b);
+        pool.push_back(r.a);
+        pool.push_back(r.b);
     }
     pending.clear();
     return rc;
@@ -466,6 +493,7 @@ int chy_ctx_destroy(chy_ctx *ctx)
     if (!ctx) return CHY_EINVAL;
     DeviceScope ds(ctx->device);
     ctx->prof.collect();
+    ctx->prof.release();
     release_workers(ctx);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->d_diag) (void)hipFree(ctx->d_diag);
@@ -634,6 +662,9 @@ int chy_ctx_profile(chy_ctx *ctx, int enable)
     DeviceScope ds(ctx->device);
     ctx->prof.reset();
     ctx->prof.on = enable != 0;
+    // events for a few hundred solves' stages, created here rather than
+    // between launches
+    if (ctx->prof.on) ctx->prof.reserve(2048);
     return CHY_OK;
 }
 

@@ -101,6 +101,7 @@ struct Profiler {
     bool on = false;
     struct Rec { int stage; hipEvent_t a, b; };
     std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;  // events ready for reuse (collect() returns them here)
     double ms[CHY_NSTAGES] = {};
     int count[CHY_NSTAGES] = {};
     int open_stage = -1;
@@ -109,6 +110,9 @@ struct Profiler {
     void end(hipStream_t s);
     int collect();  // wait for pending records and accumulate
     void reset();
+    void reserve(size_t n);  // fill the pool to n events
+    void release();          // destroy pooled events
+    hipEvent_t take();
 };
 }  // namespace chy
 
