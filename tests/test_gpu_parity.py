@@ -262,16 +262,33 @@ def test_128x64x128_against_reference(ctx):
 
 
 @pytest.mark.parametrize("jkl", [(32, 256, 32), (512, 16, 32), (16, 8, 512), (40, 8, 16), (200, 32, 64),
-                                 (20, 4, 8), (64, 64, 2)])
+                                 (20, 4, 8), (64, 64, 2), (1024, 8, 16), (16, 8, 1024), (16, 512, 16)])
 def test_grid_shapes_against_oracle(ctx, jkl):
     """Shapes that select the other kernel variants: KMAX=256 (general z
     GEMM), JMAX=512 (eight r rows per lane), LMAX=512 and LMAX<=16 transforms,
-    JMAX not a multiple of 64, KMAX=4, LMAX=2 (boundary values through PHI)."""
+    JMAX not a multiple of 64, KMAX=4, LMAX=2 (boundary values through PHI),
+    and the largest grids the library takes in each direction: JMAX=1024
+    (sixteen r rows per lane), LMAX=1024, KMAX=512.
+
+    Two oracle solves bound the error: the 80-bit one (the discrete system
+    solved closer to exactly) and the binary64 one.  The binary64 oracle's
+    own distance from the 80-bit solve grows with the grid through the
+    conditioning of the z and r operators (6e-12 at KMAX=256, 5e-11 at
+    KMAX=512, 4e-12 at JMAX=512), so the bar is relative to it: the GPU may
+    be no further from the 80-bit solve than twice what the binary64 oracle
+    is, plus 1e-13.  (At large KMAX the GPU, whose z eigenbasis is built in
+    long double, stays within 2e-14 of the 80-bit solve; at large JMAX it
+    shares the binary64 rounding of the radial coefficients and sits with
+    the binary64 oracle.)"""
     import pyoracle as po
-    po.use_extended(False)
     g = fields.unit_grid(*jkl)
     rho = fields.density_spiral_torus(g, seed=4)
     print(f"[gpu] shape {g.name}", flush=True)
+    po.use_extended(True)
+    try:
+        exact = po.solve(g, rho)
+    finally:
+        po.use_extended(False)
     want = po.solve(g, rho)
     pg = chypot.PreparedGrid(ctx, g)
     rho_d = chypot.to_device(rho)
@@ -280,9 +297,12 @@ def test_grid_shapes_against_oracle(ctx, jkl):
     ctx.sync()
     phi = chypot.to_host(phi_d)
     pg.close()
-    err = rel_max(phi, want["phi"])
-    print(f"[gpu] shape {g.name}: max rel error vs oracle {err:.2e}", flush=True)
-    assert err < 1e-11
+    err_x, err_64 = rel_max(phi, exact["phi"]), rel_max(phi, want["phi"])
+    err_o = rel_max(want["phi"], exact["phi"])
+    print(f"[gpu] shape {g.name}: max rel error vs 80-bit oracle {err_x:.2e}, vs binary64 oracle {err_64:.2e}"
+          f" (binary64 oracle vs 80-bit {err_o:.2e})", flush=True)
+    assert err_x <= 2 * err_o + 1e-13
+    assert err_64 < 1e-10
     assert abs(d.tmass - want["tmass"]) <= 1e-12 * abs(want["tmass"])
 
 
