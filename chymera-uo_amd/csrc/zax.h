@@ -1,6 +1,6 @@
-// ZAXPHI pieces shared by its standalone kernel (ghost.hip) and by the inverse
-// transform (fft.hip), which can run it on the lanes next to the axis it has
-// just produced (pot/pot3.f:311-399).
+// The pieces of ZAXPHI (pot/pot3.f:311-399) that ghost.hip's k_zaxphi is
+// built from: abscissae, the Aitken extrapolation, PHAC, and PHICHK/KLOCAT by
+// the last block to finish.
 #pragma once
 
 #include <hip/hip_runtime.h>
