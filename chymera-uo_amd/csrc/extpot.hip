@@ -15,8 +15,12 @@
 // with the C library's functions, and sqrt and the quotient are correctly
 // rounded, so the results are the host's to the bit.
 //
-// One thread per cell, J fastest: an HBM-bound pass that reads PHI and writes
-// PHI and STARPHI (24 bytes per cell).
+// An HBM-bound pass that reads PHI and writes PHI and STARPHI (24 bytes per
+// cell).  Each plane L of the hydro grid, (JMAX+2)*(KMAX+2) cells, is one
+// contiguous run; a block takes EP_CELLS consecutive cells of one plane, and
+// thread t its cells t, t+256, ..., so every load and store instruction of a
+// wavefront covers 512 contiguous bytes.  All of a thread's PHI loads are
+// issued before the first is used.
 #include "chypot_impl.h"
 
 namespace chy {
@@ -29,21 +33,40 @@ struct ExtArgs {
     double mass, ix, iy;           // MASS_STAR+TMASSACC, x*, y*
 };
 
+constexpr int EP_PER = 4;               // cells per thread
+constexpr int EP_CELLS = 256 * EP_PER;  // cells per block
+
 __global__ __launch_bounds__(256) void k_external_pot(ExtArgs a)
 {
 #pragma clang fp contract(off)
-    const int J = 1 + (int)(blockIdx.x * 64 + threadIdx.x);
-    const int K = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
-    const int L = 1 + (int)blockIdx.z;
-    if (J > a.j2 || K > a.k2) return;
+    const int L = 1 + (int)blockIdx.y;
+    const int plane = a.j2 * a.k2;
+    const int e0 = (int)blockIdx.x * EP_CELLS + (int)threadIdx.x;
     const double c = a.cs[2 * (L - 1)], s = a.cs[2 * (L - 1) + 1];
-    const double rh = a.rhf[J - 1], zh = a.zhf[K - 1];
-    const double d2 = rh * rh + zh * zh + a.ix * a.ix + a.iy * a.iy - 2.0 * rh * (c * a.ix + s * a.iy);
-    const double sp = -a.mass / sqrt(d2);
-    const size_t h = (size_t)(J - 1) + (size_t)a.j2 * ((K - 1) + (size_t)a.k2 * (L - 1));
-    const size_t q = (size_t)(J - 1) + (size_t)a.pj2 * ((K - 1) + (size_t)a.pk2 * (L - 1));
-    a.starphi[h] = sp;
-    a.phi[q] = a.phi[q] + sp;
+    const size_t hl = (size_t)plane * (L - 1), ql = (size_t)a.pj2 * a.pk2 * (L - 1);
+    size_t q[EP_PER];
+    double old[EP_PER];
+    int jm[EP_PER], km[EP_PER];
+#pragma unroll
+    for (int i = 0; i < EP_PER; ++i) {
+        const int e = e0 + 256 * i;
+        const int ec = e < plane ? e : plane - 1;  // clamped: in bounds, stored by nobody
+        km[i] = ec / a.j2;
+        jm[i] = ec - km[i] * a.j2;
+        q[i] = ql + (size_t)jm[i] + (size_t)a.pj2 * km[i];
+        old[i] = a.phi[q[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < EP_PER; ++i) {
+        const int e = e0 + 256 * i;
+        const double rh = a.rhf[jm[i]], zh = a.zhf[km[i]];
+        const double d2 = rh * rh + zh * zh + a.ix * a.ix + a.iy * a.iy - 2.0 * rh * (c * a.ix + s * a.iy);
+        const double sp = -a.mass / sqrt(d2);
+        if (e < plane) {
+            a.starphi[hl + (size_t)e] = sp;
+            a.phi[q[i]] = old[i] + sp;
+        }
+    }
 }
 
 }  // namespace
@@ -57,8 +80,8 @@ int launch_external_pot(const chy_extpot_desc *d, const double *rhf, const doubl
     x.lmax = d->lmax;
     x.mass = d->mass_star + d->tmassacc;
     x.ix = ix; x.iy = iy;
-    const dim3 grid((x.j2 + 63) / 64, (x.k2 + 3) / 4, d->lmax), block(64, 4);
-    hipLaunchKernelGGL(k_external_pot, grid, block, 0, st, x);
+    const dim3 grid((x.j2 * x.k2 + EP_CELLS - 1) / EP_CELLS, d->lmax);
+    hipLaunchKernelGGL(k_external_pot, grid, dim3(256), 0, st, x);
     return (int)hipGetLastError();
 }
 
