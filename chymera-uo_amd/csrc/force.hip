@@ -13,11 +13,15 @@
 // count); here each thread sums its cells over a run of FORCE_LC planes, the
 // block reduces its threads in a fixed tree, and one block adds the block
 // partials in block order -- a fixed order, so results repeat run to run.
+// (A last-block ticket instead of the second launch was slower: every one of
+// the ~2000 blocks then fences and increments the same counter.)
 //
 // RHO is read with the leading dimensions (JMAX+2, KMAX+2) that find_force
 // declares (pot/wiggle.F:103), as the reference reads the array it is passed.
-// One thread per (J, K) and run of planes; a wavefront reads 64 consecutive J
-// of one row per plane.  cos/sin of phi_L come from a table.
+// One thread per (J, K) and run of planes, over the flattened (J, K) range of
+// the sum with J fastest: a wavefront reads 64 consecutive cells of a plane,
+// and no block is left with a few live lanes when the row of NJ = JMAX-JMIN+5
+// cells is not a multiple of 64.  cos/sin of phi_L come from a table.
 #include "chypot_impl.h"
 
 namespace chy {
@@ -37,18 +41,35 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// the block's (fx, fy) to thread 0: wavefront sums, then the four wavefronts
+// in a fixed tree
+__device__ __forceinline__ void block_sum(double &fx, double &fy, double (&red)[2][4])
+{
+    const int t = (int)threadIdx.x;
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = fx;
+        red[1][t >> 6] = fy;
+    }
+    __syncthreads();
+    fx = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    fy = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+}
+
 constexpr int FORCE_LC = 16;  // planes L per thread
 
 __global__ __launch_bounds__(256) void k_find_force(ForceArgs a)
 {
 #pragma clang fp contract(off)
     const int t = (int)threadIdx.x;
-    const int J = a.j0 + (int)blockIdx.x * 64 + (t & 63);
-    const int K = 2 + (int)blockIdx.y * 4 + (t >> 6);
-    const int L0 = (int)blockIdx.z * FORCE_LC, L1 = min(L0 + FORCE_LC, a.lmax);
+    const int e = (int)blockIdx.x * 256 + t;  // flat (J, K) index of the sum, J fastest
+    const int L0 = (int)blockIdx.y * FORCE_LC, L1 = min(L0 + FORCE_LC, a.lmax);
     double fx = 0.0, fy = 0.0;
-    if (J < a.j0 + a.nj && K <= a.kmax + 1) {
+    if (e < a.nj * a.kmax) {
         // each cell's terms in the order of wiggle.F:119-126
+        const int kk = e / a.nj;
+        const int J = a.j0 + (e - kk * a.nj), K = 2 + kk;
         const double rh = a.rhf[J - 1], zz = a.zhf[K - 1];
         const double z2 = zz * zz;
         const size_t plane = (size_t)a.j2 * a.k2;
@@ -63,22 +84,16 @@ __global__ __launch_bounds__(256) void k_find_force(ForceArgs a)
         }
     }
     __shared__ double red[2][4];
-    fx = wave_sum(fx);
-    fy = wave_sum(fy);
-    if ((t & 63) == 0) {
-        red[0][t >> 6] = fx;
-        red[1][t >> 6] = fy;
-    }
-    __syncthreads();
+    block_sum(fx, fy, red);
     if (t == 0) {
-        const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        a.part[2 * b] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
-        a.part[2 * b + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+        const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+        a.part[2 * b] = fx;
+        a.part[2 * b + 1] = fy;
     }
 }
 
-// One block adds the partials: each thread a strided share in order, then a
-// fixed tree.
+// One block adds the partials: each thread a strided share in order, then the
+// same fixed tree.
 __global__ __launch_bounds__(256) void k_force_total(const double *part, int nparts, double *out)
 {
     const int t = (int)threadIdx.x;
@@ -88,25 +103,24 @@ __global__ __launch_bounds__(256) void k_force_total(const double *part, int npa
         fy += part[2 * i + 1];
     }
     __shared__ double red[2][4];
-    fx = wave_sum(fx);
-    fy = wave_sum(fy);
-    if ((t & 63) == 0) {
-        red[0][t >> 6] = fx;
-        red[1][t >> 6] = fy;
-    }
-    __syncthreads();
+    block_sum(fx, fy, red);
     if (t == 0) {
-        out[0] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
-        out[1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+        out[0] = fx;
+        out[1] = fy;
     }
 }
 
 }  // namespace
 
-int force_blocks(const chy_force_desc *d)
+static int force_row_blocks(const chy_force_desc *d)
 {
     const int nj = d->jmax + 1 - (d->jmin - 3) + 1;
-    return ((nj + 63) / 64) * ((d->kmax + 3) / 4) * ((d->lmax + FORCE_LC - 1) / FORCE_LC);
+    return (nj * d->kmax + 255) / 256;
+}
+
+int force_blocks(const chy_force_desc *d)
+{
+    return force_row_blocks(d) * ((d->lmax + FORCE_LC - 1) / FORCE_LC);
 }
 
 int launch_find_force(const chy_force_desc *d, const double *rhf, const double *zhf, const double *cs,
@@ -120,11 +134,11 @@ int launch_find_force(const chy_force_desc *d, const double *rhf, const double *
     a.x = x; a.y = y;
     a.vol = d->dr * d->dz * d->dphi * 2.0;
     a.gs2 = d->gsoft * d->gsoft;
-    const dim3 grid((a.nj + 63) / 64, (d->kmax + 3) / 4, (d->lmax + FORCE_LC - 1) / FORCE_LC);
+    const dim3 grid(force_row_blocks(d), (d->lmax + FORCE_LC - 1) / FORCE_LC);
     hipLaunchKernelGGL(k_find_force, grid, dim3(256), 0, st, a);
     int rc = (int)hipGetLastError();
     if (rc) return rc;
-    hipLaunchKernelGGL(k_force_total, dim3(1), dim3(256), 0, st, part, (int)(grid.x * grid.y * grid.z), out);
+    hipLaunchKernelGGL(k_force_total, dim3(1), dim3(256), 0, st, part, (int)(grid.x * grid.y), out);
     return (int)hipGetLastError();
 }
 
