@@ -73,6 +73,41 @@ def test_solve_matches_reference(ctx, name):
         assert np.abs(phase_diff(c[..., 1], rc[..., 1])[sig]).max() < 1e-6
 
 
+@pytest.mark.parametrize("npoint", range(1, 11))
+def test_every_npoint_against_oracle(ctx, npoint):
+    """ZAXPHI's extrapolation is compiled once per NPOINT (1..10): each against
+    the oracle's POT3 with the same NPOINT on a 24x8x16 spiral torus -- PHI
+    everywhere POT3 writes it (the axis column PHI(JMAX+2,K,L) is ZAXPHI's),
+    PHICHK and KLOCAT."""
+    import pyoracle as po
+    po.use_extended(False)
+    g = fields.unit_grid(24, 8, 16)
+    rho = fields.density_spiral_torus(g, seed=5)
+    print(f"[gpu] npoint {npoint} {g.name}", flush=True)
+    res = run_solve(ctx, g, rho, dict(isym=2, maxtrm=10, redge=0.0, npoint=npoint), coef_mode=chypot.COEF_NONE)
+    want = po.solve(g, rho, 10, 2, 0.0, npoint)
+    assert rel_max(res["phi"], want["phi"]) < 1e-12
+    assert abs(res["phichk"] - want["phichk"]) < 1e-10
+    assert res["klocat"] == want["klocat"]
+
+
+def test_zero_density_keeps_klocat(ctx):
+    """Zero density: PHI stays zero (no NaN from ZAXPHI's 0/0), PHICHK is zero
+    and KLOCAT keeps the value it had, as in the oracle (pot/pot3.f:380-399)."""
+    g = fields.unit_grid(24, 8, 16)
+    print(f"[gpu] zero density {g.name}", flush=True)
+    pg = chypot.PreparedGrid(ctx, g)
+    rho_d = chypot.to_device(np.zeros(g.shape, order="F"))
+    phi_d = torch.zeros_like(rho_d)
+    pg.bdygen(rho_d, phi_d, maxtrm=10, isym=2, redge=0.0)
+    d = pg.pot3(rho_d, phi_d, npoint=8, diag=chypot.Diag(0.0, 0.0, -3))
+    ctx.sync()
+    phi = chypot.to_host(phi_d)
+    pg.close()
+    assert not np.abs(phi).any()
+    assert (d.phichk, d.klocat) == (0.0, -3)
+
+
 def test_untouched_cells_and_determinism(ctx):
     g, rho, params, f = load_case("g24x16x32_spiral")
     a = run_solve(ctx, g, rho, params, phi_init=7.25)

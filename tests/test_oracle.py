@@ -174,6 +174,72 @@ def test_solution_satisfies_discrete_operator():
     assert np.abs(res[inner] / scale[inner]).max() > 1e-8
 
 
+# ------------------------------------------------------------------- ZAXPHI
+
+def axis_value_lagrange(g, phi, npoint):
+    """PHI(JMAX+2,K,L), L=1..N, as ZAXPHI defines it (pot/pot3.f:336-379): the
+    value at r=0 of the polynomial through NPOINT samples, PHI(J,K,L) at
+    x=+RHF(J) and PHI(J,K,L+N) at x=-RHF(J) for J=2..NUM+1, and for odd NPOINT
+    also PHI(NUM+2,K,L) at x=+RHF(NUM+2).  Lagrange form in numpy: the Aitken
+    recurrence of the reference is Neville's scheme for the same value."""
+    rhf, num, n = g.rhf(), npoint // 2, g.lmax // 2
+    xs = [rhf[j] for j in range(1, num + 1)] + [-rhf[j] for j in range(1, num + 1)]
+    ys = [phi[j, 1:g.kmax + 1, :n] for j in range(1, num + 1)] + \
+         [phi[j, 1:g.kmax + 1, n:] for j in range(1, num + 1)]
+    if npoint % 2:
+        xs.append(rhf[num + 1])
+        ys.append(phi[num + 1, 1:g.kmax + 1, :n])
+    w = [math.prod(-xs[m] / (xs[i] - xs[m]) for m in range(len(xs)) if m != i) for i in range(len(xs))]
+    return sum(wi * y for wi, y in zip(w, ys))
+
+
+def phichk_klocat(ph, klocat_in):
+    """PHAC(K) = the deviation 1 - PH/PHMAX of largest magnitude over L, PHMAX
+    the PH of largest magnitude (first L wins ties); PHICHK = the PHAC of
+    largest magnitude (first K wins), KLOCAT its K, or KLOCAT unchanged when
+    every PHAC is zero (pot/pot3.f:380-399).  ph is PH(K=2.., L=1..N)."""
+    phac = np.zeros(ph.shape[0])
+    for k in range(ph.shape[0]):
+        phmax = ph[k, np.argmax(np.abs(ph[k]))]
+        err = 1.0 - ph[k] / phmax
+        phac[k] = err[np.argmax(np.abs(err))]
+    i = int(np.argmax(np.abs(phac)))
+    return phac[i], (i + 2 if phac[i] != 0.0 else klocat_in)
+
+
+@pytest.mark.parametrize("npoint", range(1, 11))
+def test_zaxphi_is_the_interpolant_at_the_axis(npoint):
+    """Every NPOINT the solve accepts (1..10): the axis column POT3 writes is the
+    interpolating polynomial's value at r=0, the same value in both halves
+    L and L+N, and PHICHK/KLOCAT follow the reference's scans.  Planted in a
+    scratch copy of the oracle, reading L for the far side fails it for
+    NPOINT >= 2 and RHF(J+1) for RHF(J) for NPOINT >= 3."""
+    g = fields.unit_grid(24, 8, 16)
+    rho = fields.density_spiral_torus(g, seed=5)
+    phi_b, _ = po.bdygen(g, rho)
+    phi, _, phichk, klocat = po.pot3(g, rho, phi_b, npoint, klocat=-3)
+    n = g.lmax // 2
+    axis = phi[g.jmax + 1, 1:g.kmax + 1, :]
+    want = axis_value_lagrange(g, phi, npoint)
+    assert np.abs(axis[:, :n] - want).max() <= 1e-13 * np.abs(phi).max()
+    assert np.array_equal(axis[:, n:], axis[:, :n])
+    pc, kl = phichk_klocat(axis[:, :n], -3)
+    assert phichk == pytest.approx(pc, rel=1e-14, abs=0.0)
+    assert klocat == kl
+
+
+def test_zaxphi_keeps_klocat_for_a_zero_density():
+    """Zero density: PHI is zero, every PH is zero, 1 - PH/PHMAX is NaN and the
+    strict comparisons of pot/pot3.f:380-399 keep PHAC at zero, so PHICHK is
+    zero and KLOCAT keeps the value it had."""
+    g = fields.unit_grid(24, 8, 16)
+    rho = np.zeros(g.shape, order="F")
+    phi_b, _ = po.bdygen(g, rho)
+    phi, _, phichk, klocat = po.pot3(g, rho, phi_b, 8, klocat=-3)
+    assert not np.abs(phi).any()
+    assert (phichk, klocat) == (0.0, -3)
+
+
 def test_extended_build_matches_fp64_build():
     g, rho, p, f = load_case("c1_64x32x64")
     a = po.solve(g, rho)["phi"]
