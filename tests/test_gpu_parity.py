@@ -108,6 +108,51 @@ def test_zero_density_keeps_klocat(ctx):
     assert (d.phichk, d.klocat) == (0.0, -3)
 
 
+@pytest.mark.parametrize("path", ["pot3", "solve"])
+@pytest.mark.parametrize("mode", ["full", "k2"])
+def test_coef_where_a0_is_zero_against_oracle(ctx, path, mode):
+    """COEF on a torus with no density on J=2..9 and COEF prefilled with 5:
+    where a0 = 0 the amplitudes keep the old 5, the phases are written (720
+    degrees where a = b = 0) and COEF(J,K,N,2) = a0 = 0, as the oracle does
+    (pot/pot3.f:105-132); in K2 mode only the K=2 plane is written."""
+    import pyoracle as po
+    po.use_extended(False)
+    g = fields.unit_grid(24, 8, 16)
+    rho = fields.density_spiral_torus(g, seed=5)
+    rho[:9] = 0.0
+    rho = np.asfortranarray(rho)
+    print(f"[gpu] COEF a0=0 {g.name} {path} {mode}", flush=True)
+    phi_b, _ = po.bdygen(g, rho)
+    _, want, _, _ = po.pot3(g, rho, phi_b, 8, coef=np.full((g.jmax + 2, g.kmax + 2, g.lmax // 2, 2), 5.0,
+                                                           order="F"))
+    cm = chypot.COEF_FULL if mode == "full" else chypot.COEF_K2
+    pg = chypot.PreparedGrid(ctx, g)
+    rho_d = chypot.to_device(rho)
+    phi_d = torch.zeros_like(rho_d)
+    coef = torch.full((2, g.lmax // 2, g.kmax + 2, g.jmax + 2), 5.0, dtype=torch.float64, device="cuda")
+    if path == "pot3":
+        pg.bdygen(rho_d, phi_d, maxtrm=10, isym=2, redge=0.0)
+        pg.pot3(rho_d, phi_d, npoint=8, coef=coef, coef_mode=cm)
+    else:
+        pg.solve(rho_d, phi_d, maxtrm=10, isym=2, redge=0.0, npoint=8, coef=coef, coef_mode=cm)
+    ctx.sync()
+    got = np.asfortranarray(coef.cpu().numpy().transpose(3, 2, 1, 0))
+    pg.close()
+    if mode == "k2":
+        rest = np.ones(g.kmax + 2, dtype=bool)
+        rest[1] = False
+        assert (got[:, rest] == 5.0).all()
+        got, want = got[:, 1:2], want[:, 1:2]
+    assert np.abs(got[..., 0] - want[..., 0]).max() < 1e-11
+    exact = (want[..., 1] == 720.0) | (want[..., 0] == 5.0)
+    assert exact[1:9].any()
+    n = g.lmax // 2
+    assert (got[..., :n - 1, 1][exact[..., :n - 1]] == want[..., :n - 1, 1][exact[..., :n - 1]]).all()
+    assert np.abs(got[..., n - 1, 1] - want[..., n - 1, 1]).max() <= 1e-12 * np.abs(want[..., n - 1, 1]).max()
+    sig = (want[..., :n - 1, 0] > 1e-8) & ~exact[..., :n - 1]
+    assert np.abs(phase_diff(got[..., :n - 1, 1], want[..., :n - 1, 1])[sig]).max() < 1e-6
+
+
 def test_untouched_cells_and_determinism(ctx):
     g, rho, params, f = load_case("g24x16x32_spiral")
     a = run_solve(ctx, g, rho, params, phi_init=7.25)

@@ -174,6 +174,37 @@ def test_solution_satisfies_discrete_operator():
     assert np.abs(res[inner] / scale[inner]).max() > 1e-8
 
 
+# --------------------------------------------------------------------- COEF
+
+def coef_case():
+    """24x8x16 spiral torus with no density on J=2..9, COEF prefilled with 5."""
+    g = fields.unit_grid(24, 8, 16)
+    rho = fields.density_spiral_torus(g, seed=5)
+    rho[:9] = 0.0
+    return g, np.asfortranarray(rho), np.full((g.jmax + 2, g.kmax + 2, g.lmax // 2, 2), 5.0, order="F")
+
+
+def test_coef_keeps_amplitudes_where_a0_is_zero():
+    """COEF (pot/pot3.f:105-132) from the transform of PHI on every lane: where
+    a0 = 0 the amplitudes keep their old values while the phases are written
+    (720 degrees where a = b = 0) and COEF(J,K,N,2) = a0; elsewhere the
+    amplitudes are 2 sqrt(a^2+b^2)/|a0| and |aN|/|a0|, checked against numpy's
+    FFT of the density lane (interior lanes carry RHO, pot3.f:71-81)."""
+    g, rho, coef0 = coef_case()
+    phi_b, _ = po.bdygen(g, rho)
+    _, coef, _, _ = po.pot3(g, rho, phi_b, 8, coef=coef0)
+    n = g.lmax // 2
+    dead = coef[1:9, 1:g.kmax, :, :]  # J=2..9, K=2..KMAX: RHO = 0, a0 = 0
+    assert (dead[..., 0] == 5.0).all()
+    assert (dead[..., :n - 1, 1] == 720.0).all() and (dead[..., n - 1, 1] == 0.0).all()
+    live = rho[9:g.jmax, 1:g.kmax, :]  # J=10..JMAX, K=2..KMAX
+    f = np.fft.fft(live, axis=2)
+    amp = 2 * np.abs(f[..., 1:n]) / np.abs(f[..., :1])
+    got = coef[9:g.jmax, 1:g.kmax, :, 0]
+    assert np.abs(got[..., :n - 1] - amp).max() < 1e-12 * amp.max()
+    assert np.abs(got[..., n - 1] - np.abs(f[..., n]) / np.abs(f[..., 0])).max() < 1e-12
+
+
 # ------------------------------------------------------------------- ZAXPHI
 
 def axis_value_lagrange(g, phi, npoint):
