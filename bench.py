@@ -6,7 +6,7 @@ DELR=DELZ=1/256, seeded), plain FP64.  One solve is BDYGEN + POT3 on one
 density field, device resident, COEF for the K=2 plane.  A step is one batch
 of B independent fields (different seeds, each on its own prepared grid)
 solved by one chy_solve_batch call, which runs them as batched launches of
-up to 16 fields (default B=16: one launch per kernel); `value` is solves per
+up to 32 fields (default B=32: one launch per kernel); `value` is solves per
 second over the timed steps (B solves per step) with every density already in
 HBM.  B=1 is the single-field solve.
 
@@ -21,7 +21,7 @@ The line also carries:
     its average launch duration (`batch_stages_ms_per_launch`), one launch
     processes `fields_per_launch` fields; `roofline.traffic` is the HBM bytes
     per launch that rocprofv3 FETCH_SIZE/WRITE_SIZE recorded for that kernel
-    with launches of that many fields (profiles/potsolve_256x128x256_batch16_
+    with launches of that many fields (profiles/potsolve_256x128x256_batch<B>_
     hbm_traffic.txt, or ..._hbm_traffic.txt for single fields), or null;
   * `stages_ms`: per-stage times of single-field solves launched kernel by
     kernel (--stage-steps 0 skips them);
@@ -276,7 +276,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batch", type=int, default=16, help="independent fields per step")
+    ap.add_argument("--batch", type=int, default=32, help="independent fields per step")
     ap.add_argument("--workers", type=int, default=0, help="worker streams (default min(batch, 8))")
     ap.add_argument("--refine-steps", type=int, default=2,
                     help="correction steps of the refined path (0: do not measure it)")

@@ -203,7 +203,7 @@ enum { LANES_ALL = 0, LANES_INTERIOR = 1, LANES_BOUNDARY = 2 };
 // persistent kernel, picks field f.  Each field has the workspace of the grid
 // object it was prepared on, so fields on different grid objects run in one
 // launch without sharing scratch.  Single-field launches use nf = 1.
-constexpr int kMaxBatch = 16;
+constexpr int kMaxBatch = 32;
 struct Fields {
     int nf;
     const void *rho[kMaxBatch];  // forward input (binary64 density, or the binary32 residual)

@@ -142,7 +142,7 @@ int chy_ctx_graphs(chy_ctx *ctx, int enable);
  * for the whole batch.  Items on one grid object run in order (they share
  * its workspace).  Items on different grid objects prepared from identical
  * descriptions, with the same maxtrm, isym, redge, npoint and coef_mode, run
- * together, up to 16 per launch of each kernel; other items overlap on the
+ * together, up to 32 per launch of each kernel; other items overlap on the
  * worker streams.  Each item's results are bitwise those of chy_solve on the
  * same arguments.  If any item has a diag pointer the call waits and fills
  * every such diag.  Every item is checked before anything is launched. */

@@ -113,14 +113,14 @@ def _own_grid_job(ctx, seed, shape, coef_mode=chypot.COEF_K2, maxtrm=10, npoint=
 
 def test_batched_launches_equal_single_solves(ctx):
     """Fields on separate grid objects with one description run as one launch
-    per kernel (up to 16 fields): 18 such fields (two launch groups), fields of
+    per kernel (up to 32 fields): 34 such fields (two launch groups), fields of
     a second shape, a repeat on a grid object already in the batch (a later
     round), fields whose arguments keep them out of the groups (no COEF,
     other MAXTRM) and two fields whose NPOINT agree only once lowered to 10 --
     every one equal bit for bit to its single chy_solve, PHAC included."""
-    print("[gpu] batched launches: 18+3 fields on own grid objects, repeats, odd arguments", flush=True)
+    print("[gpu] batched launches: 34+3 fields on own grid objects, repeats, odd arguments", flush=True)
     a, b = (32, 16, 64), (16, 32, 32)
-    jobs = [_own_grid_job(ctx, 300 + i, a) for i in range(18)]
+    jobs = [_own_grid_job(ctx, 300 + i, a) for i in range(34)]
     jobs += [_own_grid_job(ctx, 400 + i, b) for i in range(3)]
     jobs.insert(5, _own_grid_job(ctx, 500, a, pg=jobs[0]["grid"]))          # same object: next round
     jobs.insert(9, _own_grid_job(ctx, 501, a, coef_mode=chypot.COEF_NONE))
@@ -158,8 +158,8 @@ def test_batched_launches_equal_single_solves(ctx):
             ctx.sync()
             for i, (jb, (phi, _, _, _)) in enumerate(zip(jobs, alone)):
                 assert torch.equal(jb["phi"].cpu(), phi), f"PHI of item {i} (async), {workers} workers"
-        # the launches the batch made: the 18 fields of shape a with NPOINT 8
-        # take two launches (16 + 2), the two with NPOINT 12 (lowered to 10)
+        # the launches the batch made: the 34 fields of shape a with NPOINT 8
+        # take two launches (32 + 2), the two with NPOINT 12 (lowered to 10)
         # and 10 one, the fields of shape b one, the fields without COEF and
         # with MAXTRM 6 one each, and the repeat on the first grid object one
         ctx.profile(True)
