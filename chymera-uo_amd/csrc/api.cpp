@@ -41,6 +41,7 @@ void set_field(Fields &F, int f, const chy_grid *w, const void *rho, double *phi
     F.massio[f] = w->d.massio;
     F.dscal[f] = w->d.dscal;
     F.ticket[f] = w->d.ticket;
+    F.zsamp[f] = w->d.zsamp;
     F.klocat_in[f] = klocat_in;
 }
 
@@ -223,7 +224,7 @@ static int run_pot3_tail(chy_grid *g, hipStream_t s, int npoint, double *phi, in
     if (p) p->end(s);
     if (rc) return rc;
     if (p) p->begin(CHY_STAGE_GHOST, s);
-    rc = dev_rc(launch_zaxphi(g, npoint, klocat_in, phi, s), "ZAXPHI launch");
+    rc = dev_rc(launch_zaxphi(g, npoint, klocat_in, phi, zaxphi_from_samples(g), s), "ZAXPHI launch");
     if (p) p->end(s);
     return rc;
 }
@@ -328,7 +329,7 @@ static int run_solve_fields(chy_grid *g, const Fields &F, hipStream_t s, int max
     if (p) p->end(s);
     if (rc) return rc;
     if (p) p->begin(CHY_STAGE_GHOST, s);
-    rc = dev_rc(launch_zaxphi_f(g, F, npoint, s), "ZAXPHI launch");
+    rc = dev_rc(launch_zaxphi_f(g, F, npoint, zaxphi_from_samples(g), s), "ZAXPHI launch");
     if (p) p->end(s);
     return rc;
 }
@@ -387,7 +388,7 @@ static int run_refined(chy_grid *g, hipStream_t s, int maxtrm, int isym, double 
             rc = dev_rc(launch_fft_inverse32(g, g->d.spec32, phi, it > 0, it == steps - 1, g->d.rstat + 2 * it, s),
                         "correction inverse FFT launch");
     }
-    if (!rc) rc = dev_rc(launch_zaxphi(g, npoint, klocat_in, phi, s), "ZAXPHI launch");
+    if (!rc) rc = dev_rc(launch_zaxphi(g, npoint, klocat_in, phi, false, s), "ZAXPHI launch");
     return rc;
 }
 
@@ -859,7 +860,7 @@ int chy_zaxphi(chy_grid *g, int npoint, double *phi_d, chy_diag *diag)
     hipStream_t s = g->ctx->stream;
     Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
     if (p) p->begin(CHY_STAGE_GHOST, s);
-    rc = dev_rc(launch_zaxphi(g, npoint, diag ? diag->klocat : 0, phi_d, s), "ZAXPHI launch");
+    rc = dev_rc(launch_zaxphi(g, npoint, diag ? diag->klocat : 0, phi_d, false, s), "ZAXPHI launch");
     if (p) p->end(s);
     if (rc == CHY_OK && diag) rc = read_diag(g, diag, false, true);
     return rc;
@@ -878,7 +879,7 @@ int chy_zaxphi_host(chy_grid *g, int npoint, double *phi_h, chy_diag *diag)
     const size_t fb = sizeof(double) * field_elems(g);
     hipError_t e = hipMemcpyAsync(g->d.h_phi, phi_h, fb, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return dev_rc((int)e, "host to device copy");
-    rc = dev_rc(launch_zaxphi(g, npoint, diag ? diag->klocat : 0, g->d.h_phi, s), "ZAXPHI launch");
+    rc = dev_rc(launch_zaxphi(g, npoint, diag ? diag->klocat : 0, g->d.h_phi, false, s), "ZAXPHI launch");
     if (rc) return rc;
     e = hipMemcpyAsync(phi_h, g->d.h_phi, fb, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);

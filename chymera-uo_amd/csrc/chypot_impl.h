@@ -89,6 +89,7 @@ struct DeviceGrid {
     double *massio = nullptr;    // [nchunk+1][2][2*kMaxTerms] exclusive prefix (in) / suffix (out)
     double *dscal = nullptr;     // [4 + KMAX+2]: tmass, phichk, klocat, _, phac[K]
     unsigned *ticket = nullptr;  // ZAXPHI blocks finished (0 between launches)
+    double *zsamp = nullptr;     // [KMAX][kZaxSamples][L]  PHI(J=2..6,K,L) as the inverse FFT wrote it
     // staging for the host-pointer entry points (allocated on first use)
     double *h_rho = nullptr, *h_phi = nullptr, *h_coef = nullptr;
 };
@@ -203,6 +204,8 @@ enum { LANES_ALL = 0, LANES_INTERIOR = 1, LANES_BOUNDARY = 2 };
 // persistent kernel, picks field f.  Each field has the workspace of the grid
 // object it was prepared on, so fields on different grid objects run in one
 // launch without sharing scratch.  Single-field launches use nf = 1.
+// ZAXPHI reads PHI(J,K,L) for J = 2..NPOINT/2+2 <= 6 (NPOINT <= 10)
+constexpr int kZaxSamples = 5;
 constexpr int kMaxBatch = 32;
 struct Fields {
     int nf;
@@ -216,6 +219,7 @@ struct Fields {
     double *massio[kMaxBatch];   // BDYGEN prefix/suffix sums
     double *dscal[kMaxBatch];    // diagnostics
     unsigned *ticket[kMaxBatch]; // ZAXPHI ticket counter
+    double *zsamp[kMaxBatch];    // ZAXPHI's samples PHI(J=2..6,K,L), copied by the inverse FFT
     int klocat_in[kMaxBatch];    // KLOCAT before the call
 };
 // one field on grid g's own workspace
@@ -254,8 +258,11 @@ int launch_mode_solve32(const chy_grid *g, float *spec32, float *yh32, hipStream
 int launch_residual(const chy_grid *g, const double *rho, const double *phi, bool first, bool clear_stat,
                     hipStream_t s);
 // ZAXPHI with PHAC/PHICHK/KLOCAT (POT3's ghost cells come with the inverse FFT).
-int launch_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s);
-int launch_zaxphi_f(const chy_grid *g, const Fields &F, int npoint, hipStream_t s);
+// samples: read PHI(J=2..6,K,L) from the copy the inverse FFT just made
+// (only where zaxphi_from_samples(g)), else from PHI itself.
+bool zaxphi_from_samples(const chy_grid *g);
+int launch_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, bool samples, hipStream_t s);
+int launch_zaxphi_f(const chy_grid *g, const Fields &F, int npoint, bool samples, hipStream_t s);
 // BDYGEN from the ring moments in d.mom: radial split sums, then either the
 // boundary values and ghosts in PHI, or (spectral) the boundary lanes'
 // azimuthal spectrum written straight into d.spec with their COEF entries

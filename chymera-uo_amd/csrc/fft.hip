@@ -107,6 +107,7 @@ struct FftArgs {
     double ys;            // factor on the interior lanes' spectrum (4 pi G)
     int ghosts;           // inverse: also write POT3's axis/midplane ghost cells
     unsigned long long *stat;  // inverse with accumulation: max |dPHI|, max |PHI| (as bits), or null
+    double *zsamp;        // inverse: [KMAX][kZaxSamples][L] copy of PHI(J=2..6,K,L) for ZAXPHI, or null
 };
 
 // lane index -> (j,k), j = J-2 (0..JMAX-1), k = K-2 (0..KMAX-1)
@@ -135,6 +136,7 @@ __device__ __forceinline__ void bind(FftArgs &a, const Fields &F, int f)
     if (a.phi_out) a.phi_out = F.phi[f];
     if (a.spec) a.spec = F.spec[f];
     if (a.mom) a.mom = F.mom[f];
+    if (a.zsamp) a.zsamp = F.zsamp[f];
     a.coef = a.coef_mode == CHY_COEF_NONE ? nullptr : F.coef[f];
 }
 
@@ -400,6 +402,11 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a, const Fields 
             const double v0 = u[r][k2].x, v1 = u[r][k2].y;
             p0[0] = v0;
             p0[plane] = v1;
+            if (!ACC && a.zsamp && j < kZaxSamples) {  // ZAXPHI's samples, contiguous in L
+                double *zp = a.zsamp + ((size_t)k * kZaxSamples + j) * L + 2 * n;
+                zp[0] = v0;
+                zp[1] = v1;
+            }
             if (mid) {
                 p0[-a.JS] = v0;
                 p0[plane - a.JS] = v1;
@@ -525,6 +532,7 @@ int launch_fft_inverse_f(const chy_grid *g, const Fields &F, hipStream_t s)
     a.spec = F.spec[0];
     a.phi_out = F.phi[0];
     a.ghosts = 1;
+    if (zaxphi_from_samples(g)) a.zsamp = F.zsamp[0];  // non-null marker: bind() takes each field's
     return dispatch<INV64>(g->N, a, F, s);
 }
 

@@ -17,6 +17,7 @@ namespace {
 
 struct GhostArgs {
     double *phi;
+    const double *zs;  // [KMAX][kZaxSamples][L] copy of PHI(J=2..6,K,L), or null: read PHI
     const double *zx;  // RHF(1..11)
     double *dscal;     // [0]=tmass [1]=phichk [2]=klocat [4+K]=PHAC(K)
     int J, K, L, JS, KS, npoint;
@@ -30,6 +31,7 @@ __global__ __launch_bounds__(256) void k_zaxphi(GhostArgs a, const Fields F)
 {
     // field f = blockIdx.y: its PHI, diagnostics and ticket
     a.phi = F.phi[blockIdx.y];
+    if (a.zs) a.zs = F.zsamp[blockIdx.y];
     a.dscal = F.dscal[blockIdx.y];
     a.ticket = F.ticket[blockIdx.y];
     a.klocat_in = F.klocat_in[blockIdx.y];
@@ -43,19 +45,31 @@ __global__ __launch_bounds__(256) void k_zaxphi(GhostArgs a, const Fields F)
     double *phi = a.phi;
 
     // every load first: the samples of this thread's L values, then the abscissae
-    // PHI(J,K,L) and PHI(J,K,L+N) for J = 2..NUM+1, and PHI(NUM+2,K,L) for odd NP
+    // PHI(J,K,L) and PHI(J,K,L+N) for J = 2..NUM+1, and PHI(NUM+2,K,L) for odd NP,
+    // from the inverse FFT's contiguous copy when there is one (one 8-byte
+    // piece per DRAM page otherwise)
     constexpr int num = (NP - NP % 2) / 2;
     double smp[PPT][2][num + 2];
+    const double *zs = a.zs ? a.zs + (size_t)(K - 2) * kZaxSamples * a.L : nullptr;
 #pragma unroll
     for (int t = 0; t < PPT; ++t) {
         const int l = tid + (int)blockDim.x * t;
         const size_t lc = l < N ? (size_t)l : 0;
+        if (zs) {
 #pragma unroll
-        for (int jj = 2; jj <= num + 1; ++jj) {
-            smp[t][0][jj - 1] = phi[row + (jj - 1) + plane * lc];
-            smp[t][1][jj - 1] = phi[row + (jj - 1) + plane * (lc + N)];
+            for (int jj = 2; jj <= num + 1; ++jj) {
+                smp[t][0][jj - 1] = zs[(size_t)(jj - 2) * a.L + lc];
+                smp[t][1][jj - 1] = zs[(size_t)(jj - 2) * a.L + lc + N];
+            }
+            if (NP % 2 == 1) smp[t][0][num + 1] = zs[(size_t)num * a.L + lc];
+        } else {
+#pragma unroll
+            for (int jj = 2; jj <= num + 1; ++jj) {
+                smp[t][0][jj - 1] = phi[row + (jj - 1) + plane * lc];
+                smp[t][1][jj - 1] = phi[row + (jj - 1) + plane * (lc + N)];
+            }
+            if (NP % 2 == 1) smp[t][0][num + 1] = phi[row + (num + 1) + plane * lc];
         }
-        if (NP % 2 == 1) smp[t][0][num + 1] = phi[row + (num + 1) + plane * lc];
     }
     double x[11];
     zax_abscissae<NP>(a.zx, x);
@@ -90,9 +104,15 @@ int launch_np(const chy_grid *g, const GhostArgs &a, const Fields &F, hipStream_
 
 }  // namespace
 
-int launch_zaxphi_f(const chy_grid *g, const Fields &F, int npoint, hipStream_t s)
+bool zaxphi_from_samples(const chy_grid *g) { return g->J >= kZaxSamples; }
+
+int launch_zaxphi_f(const chy_grid *g, const Fields &F, int npoint, bool samples, hipStream_t s)
 {
     GhostArgs a{};
+    if (samples) {
+        if (!zaxphi_from_samples(g)) return (int)hipErrorInvalidValue;
+        a.zs = F.zsamp[0];  // non-null marker: the kernel takes each field's
+    }
     a.zx = g->d.zx;
     a.J = g->J; a.K = g->K; a.L = g->L; a.JS = g->J + 2; a.KS = g->K + 2;
     a.npoint = npoint;
@@ -111,10 +131,10 @@ int launch_zaxphi_f(const chy_grid *g, const Fields &F, int npoint, hipStream_t 
     }
 }
 
-int launch_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, hipStream_t s)
+int launch_zaxphi(const chy_grid *g, int npoint, int klocat_in, double *phi, bool samples, hipStream_t s)
 {
     const Fields F = one_field(g, nullptr, phi, nullptr, klocat_in);
-    return launch_zaxphi_f(g, F, npoint, s);
+    return launch_zaxphi_f(g, F, npoint, samples, s);
 }
 
 }  // namespace chy

@@ -473,6 +473,7 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
     CHY_TRY(reserve(g, &d.massio, (size_t)(nchunk + 1) * 4 * kMaxTerms));
     CHY_TRY(reserve(g, &d.dscal, (size_t)4 + K + 2));
     CHY_TRY(reserve(g, &d.ticket, 1));
+    CHY_TRY(reserve(g, &d.zsamp, (size_t)K * kZaxSamples * L));
     return CHY_OK;
 }
 
