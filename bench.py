@@ -299,8 +299,8 @@ def main():
                          "three times (a kernel of known traffic for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the potential solve of one field runs on one MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print("bench.py measures one GPU: the potential solve of one field runs on one MI355X; "
+              "splitting a field over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import torch
