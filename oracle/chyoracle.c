@@ -15,10 +15,14 @@
  *        FFT(ISN=+1) at pot/fft.f:217-218, REALTR(+1) scaling pot/fft.f:4-14)
  *   - mode coefficients AM,BM,CM,AN,BN,CN,Y         pot/pot3.f:158-231
  *   - per-mode solve of AN X(.,k-1)+(BN(k)+T_m)X(.,k)+CN X(.,k+1)=Y, the system
- *       BLKTRI solves (pot/blktri.f:3-56).  It is solved here by an exact
- *       direct method: diagonalise the m-independent z operator (symmetrised,
- *       implicit-shift QL), then one r-tridiagonal (Thomas) solve per z
- *       eigenvalue.  Pinned against BLKTRI golden vectors.
+ *       BLKTRI solves (pot/blktri.f:3-56), two ways:
+ *       (a) an exact direct method: diagonalise the m-independent z operator
+ *           (symmetrised, implicit-shift QL), then one r-tridiagonal (Thomas)
+ *           solve per z eigenvalue;
+ *       (b) BLKTRI's own generalised cyclic reduction (pot/blktri.f:60-260)
+ *           with its root tables (COMPB/TQLRAT, :264-423), in its order of
+ *           operations: bitwise the reference's BLKTRI.
+ *       Both pinned against BLKTRI golden vectors.
  *   - inverse FFT + ghosts                          pot/pot3.f:256-298
  *   - ZAXPHI Aitken extrapolation to r=0           pot/pot3.f:311-399
  *   - SETBDY boundary list and cos/sin tables       pot/boundary.f:28-79
@@ -184,49 +188,58 @@ int ora_rfft_inv(int L, const double *A, const double *B, double *out)
 /* d[0..n-1] diagonal (out: eigenvalues), e[1..n-1] off-diagonal (destroyed),*/
 /* z n*n row-major, z[k*n+p] = component k of eigenvector p.                 */
 
-static int tridiag_ql(int n, real *d, real *e, real *z)
-{
-    for (int i = 0; i < n * n; ++i) z[i] = 0;
-    for (int i = 0; i < n; ++i) z[i * n + i] = 1;
-    for (int i = 1; i < n; ++i) e[i - 1] = e[i];
-    if (n > 0) e[n - 1] = 0;
-    for (int l = 0; l < n; ++l) {
-        int iter = 0, m;
-        for (;;) {
-            for (m = l; m < n - 1; ++m) {
-                real dd = fabs(d[m]) + fabs(d[m + 1]);
-                if (fabs(e[m]) <= REAL_EPS * dd) break;
-            }
-            if (m == l) break;
-            if (++iter > 60) return 1;
-            real g = (d[l + 1] - d[l]) / (2 * e[l]);
-            real r = hypot(g, (real)1);
-            g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
-            real s = 1, c = 1, p = 0;
-            int i;
-            for (i = m - 1; i >= l; --i) {
-                real f = s * e[i], b = c * e[i];
-                r = hypot(f, g);
-                e[i + 1] = r;
-                if (r == 0) { d[i + 1] -= p; e[m] = 0; break; }
-                s = f / r; c = g / r;
-                g = d[i + 1] - p;
-                r = (d[i] - g) * s + 2 * c * b;
-                p = s * r;
-                d[i + 1] = g + p;
-                g = c * r - b;
-                for (int k = 0; k < n; ++k) {
-                    real zf = z[k * n + i + 1];
-                    z[k * n + i + 1] = s * z[k * n + i] + c * zf;
-                    z[k * n + i] = c * z[k * n + i] - s * zf;
-                }
-            }
-            if (r == 0 && i >= l) continue;
-            d[l] -= p; e[l] = g; e[m] = 0;
-        }
+/* z may be NULL (eigenvalues only).  Instantiated for the working precision
+ * and for x87 80-bit (the exact-root variant of the cyclic reduction below). */
+#define DEFINE_TRIDIAG_QL(NAME, T, EPS)                                                      \
+    static int NAME(int n, T *d, T *e, T *z)                                                 \
+    {                                                                                        \
+        if (z) {                                                                             \
+            for (int i = 0; i < n * n; ++i) z[i] = 0;                                        \
+            for (int i = 0; i < n; ++i) z[i * n + i] = 1;                                    \
+        }                                                                                    \
+        for (int i = 1; i < n; ++i) e[i - 1] = e[i];                                         \
+        if (n > 0) e[n - 1] = 0;                                                             \
+        for (int l = 0; l < n; ++l) {                                                        \
+            int iter = 0, m;                                                                 \
+            for (;;) {                                                                       \
+                for (m = l; m < n - 1; ++m) {                                                \
+                    T dd = fabs(d[m]) + fabs(d[m + 1]);                                      \
+                    if (fabs(e[m]) <= (EPS) * dd) break;                                     \
+                }                                                                            \
+                if (m == l) break;                                                           \
+                if (++iter > 60) return 1;                                                   \
+                T g = (d[l + 1] - d[l]) / (2 * e[l]);                                        \
+                T r = hypot(g, (T)1);                                                        \
+                g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));                \
+                T s = 1, c = 1, p = 0;                                                       \
+                int i;                                                                       \
+                for (i = m - 1; i >= l; --i) {                                               \
+                    T f = s * e[i], b = c * e[i];                                            \
+                    r = hypot(f, g);                                                         \
+                    e[i + 1] = r;                                                            \
+                    if (r == 0) { d[i + 1] -= p; e[m] = 0; break; }                          \
+                    s = f / r; c = g / r;                                                    \
+                    g = d[i + 1] - p;                                                        \
+                    r = (d[i] - g) * s + 2 * c * b;                                          \
+                    p = s * r;                                                               \
+                    d[i + 1] = g + p;                                                        \
+                    g = c * r - b;                                                           \
+                    if (z)                                                                   \
+                        for (int k = 0; k < n; ++k) {                                        \
+                            T zf = z[k * n + i + 1];                                         \
+                            z[k * n + i + 1] = s * z[k * n + i] + c * zf;                    \
+                            z[k * n + i] = c * z[k * n + i] - s * zf;                        \
+                        }                                                                    \
+                }                                                                            \
+                if (r == 0 && i >= l) continue;                                              \
+                d[l] -= p; e[l] = g; e[m] = 0;                                               \
+            }                                                                                \
+        }                                                                                    \
+        return 0;                                                                            \
     }
-    return 0;
-}
+
+DEFINE_TRIDIAG_QL(tridiag_ql, real, REAL_EPS)
+DEFINE_TRIDIAG_QL(tridiag_ql_ld, long double, LDBL_EPSILON)
 
 /* ------------------------------------------------------------------------ */
 /* per-mode solve of the BLKTRI system (pot/blktri.f:3-56 contract):
@@ -337,6 +350,329 @@ int ora_mode_solve(int m, int n, const double *an, const double *bn, const doubl
     zbasis_free(&zb);
     free(ran); free(rbn); free(rcn); free(ram); free(rbm); free(rcm); free(ry);
     return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Generalised cyclic reduction of the same per-mode system, in the order    */
+/* the reference's BLKTRI applies it (pot/blktri.f:60-260 BLKTR1 and PRDCT,  */
+/* root tables of :264-344 COMPB and :348-423 TQLRAT).  N = 2^kx - 1 z rows,  */
+/* AN(1) = CN(N) = 0 (BLKTRI's NP=1 case, the only one POT3 uses).            */
+/*                                                                            */
+/* Row i (1-based) of level lv (lv = 0..kx-1) eliminates the rows between    */
+/* multiples of 2^lv.  Block b of level lv is the z sub-tridiagonal on rows   */
+/* 1 + b 2^lv .. b 2^lv + 2^(lv+1) - 1 (diagonal BN, coupling AN(i) CN(i-1)); */
+/* its roots are the negated eigenvalues, ordered by increasing magnitude.   */
+/* Level 0 blocks are single rows with root -BN(i).                          */
+
+typedef struct {
+    int n, kx;
+    double **lev;   /* lev[lv] + b * (2^(lv+1) - 1): roots of block b of level lv */
+} cr_roots;
+
+/* How the roots and the r solves are formed (the variants measure which part
+ * of BLKTRI's rounding its result carries):
+ *   roots: 0 = rational QL with the reference's tolerance (100 x the first
+ *              power of ten below binary64 resolution), as TQLRAT;
+ *          1 = eigenvalues of the same sub-tridiagonals by implicit QL in
+ *              x87 80-bit arithmetic, rounded once;
+ *   rsolve: 0 = elimination from the last r row up, substitution down (the
+ *               order of pot/blktri.f:208-231);
+ *           1 = elimination from the first row down, substitution up.      */
+static int cr_roots_kind = 0, cr_rsolve_kind = 0;
+
+void ora_cr_variant(int roots, int rsolve) { cr_roots_kind = roots; cr_rsolve_kind = rsolve; }
+
+/* Mode solver of ora_pot3: 0 = z eigenbasis (default; in the 80-bit build the
+ * exact reference point), 1 = the cyclic reduction below (binary64 in both
+ * builds: BLKTRI's own arithmetic). */
+static int ora_modes = 0;
+
+void ora_mode_solver(int kind) { ora_modes = kind; }
+
+/* the reference's convergence scale: 100 x the first 10^-k with 1 + 10^-k == 1 */
+static double cr_machep(void)
+{
+    volatile double e = 1, one_plus;
+    do {
+        e = e / 10.;
+        one_plus = 1. + e;
+    } while (one_plus > 1.);
+    return 100. * e;
+}
+
+/* Eigenvalues of the symmetric tridiagonal with diagonal d[0..n-1] and squared
+ * off-diagonals e2[0..n-2] (e2[i] couples rows i and i+1) by the rational QL
+ * iteration without square roots of the off-diagonals (Reinsch's algorithm, as
+ * EISPACK TQLRAT; pot/blktri.f:348-423).  On return d holds the eigenvalues in
+ * order of increasing magnitude.  e2 is destroyed (length n). */
+static int rational_ql(int n, double *d, double *e2, double machep)
+{
+    if (n == 1) return 0;
+    e2[n - 1] = 0;
+    double f = 0, b = 0, c = 0;
+    for (int l = 0; l < n; ++l) {
+        double h = machep * (fabs(d[l]) + sqrt(e2[l]));
+        if (!(b > h)) { b = h; c = b * b; }
+        int m = l;
+        while (!(e2[m] <= c)) ++m;
+        if (m != l) {
+            for (int it = 0;; ++it) {
+                if (it == 30) return l + 1;
+                double s = sqrt(e2[l]), g = d[l];
+                double p = (d[l + 1] - g) / (2.0 * s);
+                double r = sqrt(p * p + 1.0);
+                d[l] = s / (p + copysign(r, p));
+                h = g - d[l];
+                for (int i = l + 1; i < n; ++i) d[i] = d[i] - h;
+                f = f + h;
+                g = d[m];
+                if (g == 0.0) g = b;
+                h = g;
+                s = 0.0;
+                for (int i = m - 1; i >= l; --i) {
+                    p = g * h;
+                    r = p + e2[i];
+                    e2[i + 1] = s * r;
+                    s = e2[i] / r;
+                    d[i + 1] = h + s * (h + d[i]);
+                    g = d[i] - e2[i] / g;
+                    if (g == 0.0) g = b;
+                    h = g * p / r;
+                }
+                e2[l] = s * g;
+                d[l] = h;
+                if (h == 0.0) break;
+                if (fabs(e2[l]) <= fabs(c / h)) break;
+                e2[l] = h * e2[l];
+                if (e2[l] == 0.0) break;
+            }
+        }
+        double p = d[l] + f;
+        int i = l;
+        while (i > 0 && !(fabs(p) >= fabs(d[i - 1]))) { d[i] = d[i - 1]; --i; }
+        d[i] = p;
+    }
+    return 0;
+}
+
+static void cr_roots_free(cr_roots *t)
+{
+    if (t->lev)
+        for (int lv = 0; lv < t->kx; ++lv) free(t->lev[lv]);
+    free(t->lev);
+    t->lev = NULL;
+}
+
+/* an, bn, cn: the z coefficients (0-based, an[0] = cn[n-1] = 0) */
+static int cr_roots_init(cr_roots *t, int n, const double *an, const double *bn, const double *cn)
+{
+    int kx = 0;
+    while ((1 << kx) - 1 < n) ++kx;
+    if ((1 << kx) - 1 != n || kx < 2) return 2;
+    t->n = n; t->kx = kx;
+    t->lev = calloc(kx, sizeof(double *));
+    /* squared symmetrised couplings, as COMPB forms them: sqrt(AN(i) CN(i-1)) squared */
+    double *cpl2 = malloc(sizeof(double) * n);
+    for (int i = 1; i < n; ++i) {
+        double arg = an[i] * cn[i - 1];
+        if (arg < 0) { free(cpl2); cr_roots_free(t); return 4; }
+        double e = sqrt(arg);
+        cpl2[i] = e * e;
+    }
+    double machep = cr_machep();
+    t->lev[0] = malloc(sizeof(double) * n);
+    for (int i = 0; i < n; ++i) t->lev[0][i] = -bn[i];
+    for (int lv = 1; lv < kx; ++lv) {
+        int step = 1 << lv, size = 2 * step - 1, nblk = (1 << (kx - lv)) - 1;
+        t->lev[lv] = malloc(sizeof(double) * size * nblk);
+        double *d = malloc(sizeof(double) * size), *e2 = malloc(sizeof(double) * size);
+        for (int b = 0; b < nblk; ++b) {
+            int r0 = b * step; /* 0-based first row */
+            for (int i = 0; i < size; ++i) d[i] = bn[r0 + i];
+            for (int i = 0; i + 1 < size; ++i) e2[i] = cpl2[r0 + i + 1];
+            int rc = 0;
+            if (cr_roots_kind == 0) {
+                rc = rational_ql(size, d, e2, machep);
+            } else {
+                long double *dd = malloc(sizeof(long double) * size);
+                long double *ee = malloc(sizeof(long double) * (size + 1));
+                long double *zz = malloc(sizeof(long double) * size * size);
+                for (int i = 0; i < size; ++i) dd[i] = bn[r0 + i];
+                for (int i = 1; i < size; ++i) ee[i] = sqrtl((long double)an[r0 + i] * cn[r0 + i - 1]);
+                /* eigenvalues only are needed; the 80-bit QL of tridiag_ql is reused */
+                rc = tridiag_ql_ld(size, dd, ee, zz);
+                for (int i = 0; i < size; ++i) d[i] = (double)dd[i];
+                free(dd); free(ee); free(zz);
+                /* order of increasing magnitude, as the reference applies them */
+                for (int i = 1; i < size; ++i) {
+                    double v = d[i];
+                    int k = i;
+                    while (k > 0 && fabs(v) < fabs(d[k - 1])) { d[k] = d[k - 1]; --k; }
+                    d[k] = v;
+                }
+            }
+            if (rc) { free(d); free(e2); cr_roots_free(t); return 4; }
+            for (int i = 0; i < size; ++i) t->lev[lv][b * size + i] = -d[i];
+        }
+        free(d); free(e2);
+    }
+    free(cpl2);
+    return 0;
+}
+
+static const double *cr_blk(const cr_roots *t, int lv, int b)
+{
+    return t->lev[lv] + (size_t)b * ((2 << lv) - 1);
+}
+
+/* (T - rt I) w = y along r, T = tridiag(am, bm, cm) of order m */
+static void cr_rsolve(int m, const double *am, const double *bm, const double *cm, double rt,
+                      const double *y, double *w, double *d)
+{
+    if (cr_rsolve_kind == 0) {
+        d[m - 1] = am[m - 1] / (bm[m - 1] - rt);
+        w[m - 1] = y[m - 1] / (bm[m - 1] - rt);
+        for (int i = m - 2; i >= 1; --i) {
+            double den = bm[i] - rt - cm[i] * d[i + 1];
+            d[i] = am[i] / den;
+            w[i] = (y[i] - cm[i] * w[i + 1]) / den;
+        }
+        double den = bm[0] - rt - cm[0] * d[1];
+        w[0] = (den == 0) ? 1. : (y[0] - cm[0] * w[1]) / den;
+        for (int i = 1; i < m; ++i) w[i] = w[i] - d[i] * w[i - 1];
+    } else {
+        d[0] = cm[0] / (bm[0] - rt);
+        w[0] = y[0] / (bm[0] - rt);
+        for (int i = 1; i < m - 1; ++i) {
+            double den = bm[i] - rt - am[i] * d[i - 1];
+            d[i] = cm[i] / den;
+            w[i] = (y[i] - am[i] * w[i - 1]) / den;
+        }
+        double den = bm[m - 1] - rt - am[m - 1] * d[m - 2];
+        w[m - 1] = (y[m - 1] - am[m - 1] * w[m - 2]) / den;
+        for (int i = m - 2; i >= 0; --i) w[i] = w[i] - d[i] * w[i + 1];
+    }
+}
+
+/* y = sgn [prod (T - mult)] [prod (T - div)^-1] x with scalar factors aa
+ * interleaved (pot/blktri.f:177-260).  Divisors are taken from the end of
+ * div; after each division one multiplier root is folded in as
+ * y + (rt - mult) w = (T - mult) w unless the next divisor is closer to it.
+ * With na > 0 (no multipliers) each division is preceded by a factor
+ * aa[ia], ia = na-1 down to 0.  work: 2m doubles. */
+static void cr_product(int m, const double *am, const double *bm, const double *cm,
+                       int nd, const double *div, int n1, const double *mult1, int n2,
+                       const double *mult2, int na, const double *aa, const double *x,
+                       double *y, double *w, double *work, int sgn)
+{
+    double *dw = work;
+    if (nd > 0 && sgn <= 0)
+        for (int i = 0; i < m; ++i) { w[i] = -x[i]; y[i] = w[i]; }
+    else
+        for (int i = 0; i < m; ++i) { w[i] = x[i]; y[i] = x[i]; }
+    int id = nd, ia = na, m1 = n1, m2 = n2, deferred = 0;
+    for (;;) {
+        if (ia > 0) {
+            double f = aa[--ia];
+            for (int i = 0; i < m; ++i) y[i] = f * w[i];
+        }
+        if (id <= 0) return;
+        double rt = div[--id];
+        if (id == 0) deferred = 1;
+        cr_rsolve(m, am, bm, cm, rt, y, w, dw);
+        if (na > 0) continue;
+        const double *mr = NULL;
+        if (m1 > 0 && (m2 <= 0 || !(fabs(mult1[m1 - 1]) <= fabs(mult2[m2 - 1])))) {
+            if (deferred || !(fabs(mult1[m1 - 1] - div[id - 1]) < fabs(mult1[m1 - 1] - rt)))
+                mr = &mult1[--m1];
+        } else if (m2 > 0) {
+            if (deferred || !(fabs(mult2[m2 - 1] - div[id - 1]) < fabs(mult2[m2 - 1] - rt)))
+                mr = &mult2[--m2];
+        }
+        if (!mr) {
+            for (int i = 0; i < m; ++i) y[i] = w[i];
+            deferred = 1;
+            continue;
+        }
+        rt = rt - *mr;
+        for (int i = 0; i < m; ++i) y[i] = y[i] + rt * w[i];
+    }
+}
+
+/* y is Y(M,N) column-major (r fastest), overwritten by X.  cnp is CN shifted
+ * one row down (cnp[i] = cn[i-1], cnp[0] = cn[n-1] = 0), the form BLKTRI
+ * hands its reduction (pot/blktri.f:46-51). */
+static void cr_solve(const cr_roots *t, int m, const double *an, const double *cnp,
+                     const double *am, const double *bm, const double *cm, double *y)
+{
+    int n = t->n, kx = t->kx;
+    double *p = malloc(sizeof(double) * (size_t)m * (n + 1));
+    double *w1 = malloc(sizeof(double) * m), *w2 = malloc(sizeof(double) * m);
+    double *ww = malloc(sizeof(double) * m), *wk = malloc(sizeof(double) * 2 * m);
+#define YC(i) (y + (size_t)((i) - 1) * m)
+#define PC(i) (p + (size_t)(i) * m)
+    /* reduction (pot/blktri.f:72-120) */
+    for (int lv = 0; lv < kx - 1; ++lv) {
+        int h = 1 << lv, iim = h - 1, iiz = 2 * h - 1;
+        int sg = (lv & 1) ? -1 : 1;
+        for (int o = h; o <= n; o += 2 * h) {
+            int b = o / h - 1;
+            cr_product(m, am, bm, cm, iiz, cr_blk(t, lv, b), iim, lv ? cr_blk(t, lv - 1, 2 * b) : NULL,
+                       iim, lv ? cr_blk(t, lv - 1, 2 * b + 2) : NULL, 0, NULL, YC(o), PC(o), ww, wk, sg);
+        }
+        for (int e = 2 * h; e + h <= n; e += 2 * h) {
+            int s = e / (2 * h);
+            cr_product(m, am, bm, cm, iim, lv ? cr_blk(t, lv - 1, 4 * s - 2) : NULL, 0, NULL, 0, NULL,
+                       h, an + (e - h), PC(e - h), w1, ww, wk, -sg);
+            cr_product(m, am, bm, cm, iim, lv ? cr_blk(t, lv - 1, 4 * s) : NULL, 0, NULL, 0, NULL,
+                       h, cnp + e, PC(e + h), w2, ww, wk, -sg);
+            double *ye = YC(e);
+            for (int i = 0; i < m; ++i) ye[i] = w1[i] + w2[i] - ye[i];
+        }
+    }
+    /* back substitution (pot/blktri.f:122-170) */
+    for (int lv = kx - 1; lv >= 0; --lv) {
+        int h = 1 << lv, iim = h - 1, iiz = 2 * h - 1;
+        int sg = (lv & 1) ? -1 : 1, nrow = (1 << (kx - lv)) - 1;
+        for (int jj = 1; jj <= nrow; jj += 2) {
+            int i = jj * h;
+            if (jj > 1)
+                cr_product(m, am, bm, cm, iim, lv ? cr_blk(t, lv - 1, 2 * jj - 2) : NULL, 0, NULL, 0,
+                           NULL, h, an + (i - h), YC(i - h), w1, ww, wk, -sg);
+            else
+                for (int r = 0; r < m; ++r) w1[r] = 0;
+            if (jj < nrow)
+                cr_product(m, am, bm, cm, iim, lv ? cr_blk(t, lv - 1, 2 * jj) : NULL, 0, NULL, 0, NULL,
+                           h, cnp + i, YC(i + h), w2, ww, wk, -sg);
+            else
+                for (int r = 0; r < m; ++r) w2[r] = 0;
+            double *yi = YC(i);
+            for (int r = 0; r < m; ++r) w1[r] = yi[r] - w1[r] - w2[r];
+            cr_product(m, am, bm, cm, iiz, cr_blk(t, lv, jj - 1), iim,
+                       lv ? cr_blk(t, lv - 1, 2 * jj - 2) : NULL, iim, lv ? cr_blk(t, lv - 1, 2 * jj) : NULL,
+                       0, NULL, w1, yi, ww, wk, sg);
+        }
+    }
+#undef YC
+#undef PC
+    free(p); free(w1); free(w2); free(ww); free(wk);
+}
+
+/* BLKTRI's system solved by the cyclic reduction above (binary64 always) */
+int ora_cr_solve(int m, int n, const double *an, const double *bn, const double *cn,
+                 const double *am, const double *bm, const double *cm, double *y)
+{
+    cr_roots t = {0};
+    if (m < 2 || n < 3) return 1;
+    int rc = cr_roots_init(&t, n, an, bn, cn);
+    if (rc) return rc;
+    double *cnp = malloc(sizeof(double) * n);
+    cnp[0] = cn[n - 1];
+    for (int i = 1; i < n; ++i) cnp[i] = cn[i - 1];
+    cr_solve(&t, m, an, cnp, am, bm, cm, y);
+    free(cnp);
+    cr_roots_free(&t);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -517,6 +853,25 @@ int ora_pot3(int jmax, int kmax, int lmax, const double *r, const double *z, con
     zbasis zb;
     int rc = zbasis_init(&zb, NZ, pc.an, pc.bn, pc.cn);
     if (rc) { zbasis_free(&zb); pot3_coef_free(&pc); free(phi); return rc; }
+    /* or BLKTRI's cyclic reduction (binary64, the reference's arithmetic) */
+    cr_roots crt = {0};
+    double *cr_an = NULL, *cr_cnp = NULL, *cr_am = NULL, *cr_cm = NULL;
+    if (ora_modes == 1) {
+        cr_an = malloc(sizeof(double) * NZ); cr_cnp = malloc(sizeof(double) * NZ);
+        double *bn = malloc(sizeof(double) * NZ), *cn = malloc(sizeof(double) * NZ);
+        cr_am = malloc(sizeof(double) * M); cr_cm = malloc(sizeof(double) * M);
+        for (int k = 0; k < NZ; ++k) { cr_an[k] = (double)pc.an[k]; bn[k] = (double)pc.bn[k]; cn[k] = (double)pc.cn[k]; }
+        for (int j = 0; j < M; ++j) { cr_am[j] = (double)pc.am[j]; cr_cm[j] = (double)pc.cm[j]; }
+        cr_cnp[0] = cn[NZ - 1];
+        for (int k = 1; k < NZ; ++k) cr_cnp[k] = cn[k - 1];
+        rc = cr_roots_init(&crt, NZ, cr_an, bn, cn);
+        free(bn); free(cn);
+        if (rc) {
+            free(cr_an); free(cr_cnp); free(cr_am); free(cr_cm);
+            zbasis_free(&zb); pot3_coef_free(&pc); free(phi);
+            return rc;
+        }
+    }
     real pig4 = 4 * (real)pi * (real)grav;
 #pragma omp parallel
     {
@@ -539,7 +894,16 @@ int ora_pot3(int jmax, int kmax, int lmax, const double *r, const double *z, con
             for (int j = 2; j <= jmax - 2; ++j)
                 bm[j - 1] = -pc.cm[j - 1] - pc.am[j - 1] + 2 * (xlam - 1) * pc.c[j - 1];
             bm[jmax - 2] = -pc.cmmax - pc.am[jmax - 2] + 2 * (xlam - 1) * pc.c[jmax - 2];
-            mode_solve(&zb, M, pc.am, bm, pc.cm, y, work);
+            if (ora_modes == 1) {
+                double *yd = malloc(sizeof(double) * (size_t)M * NZ), *bmd = malloc(sizeof(double) * M);
+                for (size_t i = 0; i < (size_t)M * NZ; ++i) yd[i] = (double)y[i];
+                for (int j = 0; j < M; ++j) bmd[j] = (double)bm[j];
+                cr_solve(&crt, M, cr_an, cr_cnp, cr_am, bmd, cr_cm, yd);
+                for (size_t i = 0; i < (size_t)M * NZ; ++i) y[i] = yd[i];
+                free(yd); free(bmd);
+            } else {
+                mode_solve(&zb, M, pc.am, bm, pc.cm, y, work);
+            }
             for (int k = 2; k <= kmax; ++k)
                 for (int j = 2; j <= jmax; ++j)
                     phi[P3(g, j, k, l)] = y[(size_t)(j - 2) + (size_t)M * (k - 2)];
@@ -548,6 +912,10 @@ int ora_pot3(int jmax, int kmax, int lmax, const double *r, const double *z, con
     }
     zbasis_free(&zb);
     pot3_coef_free(&pc);
+    if (ora_modes == 1) {
+        cr_roots_free(&crt);
+        free(cr_an); free(cr_cnp); free(cr_am); free(cr_cm);
+    }
 
     /* inverse transform (pot3.f:256-283) */
 #pragma omp parallel

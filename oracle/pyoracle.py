@@ -36,6 +36,9 @@ def lib():
         L.ora_rfft_fwd.argtypes = [C.c_int, d, d, d]
         L.ora_rfft_inv.argtypes = [C.c_int, d, d, d]
         L.ora_mode_solve.argtypes = [C.c_int, C.c_int, d, d, d, d, d, d, d]
+        L.ora_cr_solve.argtypes = [C.c_int, C.c_int, d, d, d, d, d, d, d]
+        L.ora_cr_variant.argtypes = [C.c_int, C.c_int]
+        L.ora_mode_solver.argtypes = [C.c_int]
         grid = [C.c_int, C.c_int, C.c_int, d, d, d, d, C.c_double, C.c_double, C.c_double]
         L.ora_pot3.argtypes = grid + [C.c_int, d, d, d, d, i]
         L.ora_bdygen.argtypes = grid + [C.c_int, C.c_int, C.c_double, d, d, d]
@@ -96,6 +99,35 @@ def mode_solve(an, bn, cn, am, bm, cm, y):
     return y
 
 
+def cr_solve(an, bn, cn, am, bm, cm, y, roots="reference", order="reference"):
+    """The same system by BLKTRI's cyclic reduction (pot/blktri.f:60-260).
+    roots: "reference" (TQLRAT's rule, the reference's values) or "exact"
+    (80-bit eigenvalues rounded once); order: "reference" (r solves eliminate
+    from the last row up) or "other" (from the first row down)."""
+    y = np.array(y, dtype=np.float64, order="C", copy=True)
+    n, m = y.shape
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (an, bn, cn, am, bm, cm)]
+    L = lib()
+    L.ora_cr_variant(0 if roots == "reference" else 1, 0 if order == "reference" else 1)
+    try:
+        rc = L.ora_cr_solve(m, n, *[_p(a) for a in args], _p(y))
+    finally:
+        L.ora_cr_variant(0, 0)
+    if rc:
+        raise ValueError(f"ora_cr_solve rc={rc}")
+    return y
+
+
+_modes = 0
+
+
+def use_cyclic_reduction(flag: bool):
+    """POT3's mode solves (solve(), pot3()) by BLKTRI's cyclic reduction, the
+    reference's own arithmetic (True), or by the z eigenbasis (False, default)."""
+    global _modes
+    _modes = 1 if flag else 0
+
+
 def _grid_args(g):
     r = np.ascontiguousarray(g.r())
     z = np.ascontiguousarray(g.z())
@@ -129,6 +161,7 @@ def pot3(g, rho, phi, npoint=8, coef=None, klocat=0):
     pc = np.zeros(1)
     kl = C.c_int(klocat)
     keep, ga = _grid_args(g)
+    lib().ora_mode_solver(_modes)
     rc = lib().ora_pot3(*ga, npoint, _p(rho), _p(phi), _p(coef), _p(pc), C.byref(kl))
     if rc:
         raise ValueError(f"ora_pot3 rc={rc}")

@@ -252,8 +252,16 @@ def make_large(grid=(256, 128, 256), nrep=0):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle
     pyoracle.use_extended(True)
-    ext = pyoracle.solve(g, rho)["phi"]
-    pyoracle.use_extended(False)
+    try:
+        ext = pyoracle.solve(g, rho)["phi"]
+        # the discrete operator's residual of both fields, evaluated in 80-bit
+        # arithmetic: it arbitrates which one is closer to the exact solution
+        rref, scale = pyoracle.residual(g, phi, rho)
+        rext, _ = pyoracle.residual(g, ext, rho)
+    finally:
+        pyoracle.use_extended(False)
+    # whole-field statistics over every cell POT3 writes (K = 1..KMAX+1)
+    wr = phi[:, :g.kmax + 1, :]
     save = dict(grid=np.array(grid, dtype=np.int64), spacing=np.array([g.delr, g.delz]),
                 phi_k2=phi[:, 1, :], phi_l1=phi[:, :, 0],
                 phi_absmax=np.array(np.abs(phi).max()), phi_sum=np.array(phi.sum()),
@@ -262,7 +270,11 @@ def make_large(grid=(256, 128, 256), nrep=0):
                 tmass=np.array(res["tmass"]), phichk=np.array(res["phichk"]),
                 coef_k2=res["coef"][:, 1, :8, :],
                 ext_phi_k2=ext[:, 1, :], ext_phi_l1=ext[:, :, 0],
-                ext_ref_maxdiff=np.array(np.abs(ext - phi).max() / np.abs(ext).max()))
+                ext_ref_maxdiff=np.array(np.abs(ext - phi).max() / np.abs(ext).max()),
+                phi_kmax=np.abs(wr).max(axis=(0, 2)), phi_lmax=np.abs(wr).max(axis=(0, 1)),
+                phi_ksum=wr.sum(axis=(0, 2)), phi_lsum=wr.sum(axis=(0, 1)),
+                ref_resid=np.array(np.abs(rref).max() / scale.max()),
+                ext_resid=np.array(np.abs(rext).max() / scale.max()))
     name = f"large_{g.name}"
     np.savez_compressed(GOLDEN / f"solve_{name}.npz", **save)
     mpath = GOLDEN / "solve_meta.json"

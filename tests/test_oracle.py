@@ -74,6 +74,76 @@ def test_mode_solve_matches_reference_blktri():
         assert rel_max(x, e[f"case{i}_x"]) < 1e-13, (i, g.name)
 
 
+def test_cyclic_reduction_is_the_reference_blktri_bitwise():
+    """The oracle's restatement of BLKTRI's generalised cyclic reduction
+    (pot/blktri.f:60-423: COMPB/TQLRAT root tables, BLKTR1's reduction and
+    back substitution, PRDCT's root order and multiplier choice) gives the
+    reference's own BLKTRI results bit for bit on every golden system.  A
+    root table converged to another tolerance, another root order or another
+    elimination order all change the last bits."""
+    e = np.load(GOLDEN / "blktri_modes.npz")
+    ncase = len([k for k in e.files if k.endswith("_grid")])
+    for i in range(ncase):
+        jk = e[f"case{i}_grid"]
+        g = fields.Grid(int(jk[0]), int(jk[1]), int(jk[2]), float(jk[3]), float(jk[4]))
+        x = po.cr_solve(*mg.pot3_coeffs(g, int(jk[5])), e[f"case{i}_y"])
+        assert np.array_equal(x, e[f"case{i}_x"]), (i, g.name, rel_max(x, e[f"case{i}_x"]))
+        # the variants are not: eliminating the r systems from the first row
+        # down moves the result by rounding only
+        x2 = po.cr_solve(*mg.pot3_coeffs(g, int(jk[5])), e[f"case{i}_y"], order="other")
+        assert rel_max(x2, e[f"case{i}_x"]) < 1e-13
+
+
+@pytest.mark.parametrize("grid,m", [((16, 8, 16), 0), ((24, 16, 32), 7), ((40, 32, 64), 1),
+                                    ((20, 16, 64), 31), ((64, 64, 16), 2)])
+def test_cyclic_reduction_solves_the_mode_system(grid, m):
+    """Both root tables solve the same 2-D system as a sparse direct solve."""
+    g = fields.Grid(grid[0], grid[1], grid[2], 0.9 / grid[0], 0.55 / grid[1])
+    co = mg.pot3_coeffs(g, m)
+    A = _mode_matrix(*co)
+    y = np.random.default_rng(m).standard_normal((grid[1] - 1, grid[0] - 1))
+    x_sp = spla.spsolve(A, y.ravel()).reshape(y.shape)
+    assert rel_max(po.cr_solve(*co, y), x_sp) < 1e-12
+    assert rel_max(po.cr_solve(*co, y, roots="exact"), x_sp) < 1e-12
+
+
+def test_reference_error_at_kmax128_comes_from_its_root_table():
+    """At KMAX=128 BLKTRI's result is ~1e-12 from the exact solution of its
+    system.  Its root table (TQLRAT's convergence rule) makes that error: the
+    same reduction with exact roots is 10x closer to the exact solution, and
+    with the reference's roots but another elimination order it stays within
+    ~1e-14 of the reference's result."""
+    g = fields.unit_grid(256, 128, 16)
+    co = mg.pot3_coeffs(g, 1)
+    y = np.random.default_rng(5).uniform(-1, 1, (127, 255))
+    po.use_extended(True)
+    try:
+        exact = po.mode_solve(*co, y)
+    finally:
+        po.use_extended(False)
+    ref_roots = po.cr_solve(*co, y)
+    other = po.cr_solve(*co, y, order="other")
+    ex_roots = po.cr_solve(*co, y, roots="exact")
+    e_ref = rel_max(ref_roots, exact)
+    assert e_ref > 5e-13
+    assert rel_max(ex_roots, exact) < 0.1 * e_ref
+    assert rel_max(other, ref_roots) < 2e-14
+
+
+def test_pot3_by_cyclic_reduction_matches_reference():
+    """The oracle's POT3 with the cyclic reduction: the reference's whole solve
+    on the golden grids to well inside the 1e-12 bar."""
+    po.use_cyclic_reduction(True)
+    try:
+        for name in SMALL_CASES:
+            g, rho, params, f = load_case(name)
+            phi, _ = po.bdygen(g, rho, None, params["maxtrm"], params["isym"], params["redge"])
+            phi, coef, phichk, klocat = po.pot3(g, rho, phi, params["npoint"])
+            assert rel_max(phi, f["phi"]) < 1e-13, name
+    finally:
+        po.use_cyclic_reduction(False)
+
+
 def _mode_matrix(an, bn, cn, am, bm, cm):
     """The 2-D system BLKTRI solves (pot/blktri.f:3-56), unknown (i,k) at k*M+i."""
     M, N = len(am), len(an)
