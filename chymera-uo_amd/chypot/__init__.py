@@ -25,12 +25,17 @@ LIB_PATH = _HERE.parent / "_lib" / "libchypot.so"
 
 CHY_OK, CHY_EINVAL, CHY_EUNSUPPORTED, CHY_ENOMEM, CHY_EDEVICE, CHY_ESTATE, CHY_ENUMERIC = range(7)
 COEF_NONE, COEF_K2, COEF_FULL = 0, 1, 2
-STAGES = ("bdygen", "fft_fwd", "zfwd", "rsolve", "zinv", "fft_inv", "ghost", "fft_bdy")
+STAGES = ("bdygen", "fft_fwd", "zfwd", "rsolve", "zinv", "fft_inv", "ghost", "fft_bdy", "cr")
+# per-mode solver of a grid (chy_grid_set_mode_solver)
+MODES_CR, MODES_EIGEN = 0, 1
+# tunable kernel shapes (chy_grid_set_tune)
+TUNE_CR_LANES = 0
 
 # every entry point declared in include/chypot.h
 EXPORTS = (
     "chy_ctx_create", "chy_ctx_destroy", "chy_ctx_set_stream", "chy_ctx_sync",
     "chy_setbdy", "chy_grid_destroy", "chy_grid_device_bytes",
+    "chy_grid_set_mode_solver", "chy_grid_mode_solver", "chy_grid_set_tune", "chy_grid_tune",
     "chy_bdygen", "chy_pot3", "chy_solve", "chy_bdygen_host", "chy_pot3_host",
     "chy_ctx_profile", "chy_ctx_stage_times", "chy_ctx_graphs",
     "chy_zaxphi", "chy_zaxphi_host", "chy_phac", "chy_solve_batch", "chy_ctx_workers",
@@ -103,6 +108,10 @@ def lib():
         L.chy_grid_destroy.argtypes = [vp]
         L.chy_grid_device_bytes.argtypes = [vp]
         L.chy_grid_device_bytes.restype = C.c_size_t
+        L.chy_grid_set_mode_solver.argtypes = [vp, C.c_int]
+        L.chy_grid_mode_solver.argtypes = [vp]
+        L.chy_grid_set_tune.argtypes = [vp, C.c_int, C.c_int]
+        L.chy_grid_tune.argtypes = [vp, C.c_int]
         L.chy_bdygen.argtypes = [vp, C.c_int, C.c_int, C.c_double, vp, vp, C.POINTER(Diag)]
         L.chy_pot3.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int, C.POINTER(Diag)]
         L.chy_solve.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp, C.c_int,
@@ -134,12 +143,18 @@ def _check(rc: int, where: str):
         raise ChyError(rc, where)
 
 
-def _dptr(t):
-    """Device pointer of a torch CUDA tensor (float64, contiguous) or None."""
+def _dptr(t, shape=None, device=None, name="tensor"):
+    """Device pointer of a torch CUDA tensor (float64, contiguous) or None.
+    shape/device: what the kernels will index (checked, so that a wrong-sized
+    or misplaced tensor is an error here rather than an out-of-bounds access)."""
     if t is None:
         return None
     if not t.is_cuda or t.dtype.__repr__() != "torch.float64" or not t.is_contiguous():
-        raise ValueError("expected a contiguous float64 CUDA tensor")
+        raise ValueError(f"{name}: expected a contiguous float64 CUDA tensor")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if device is not None and t.device.index != device:
+        raise ValueError(f"{name}: on cuda:{t.device.index}, the context is on cuda:{device}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -183,8 +198,10 @@ class Context:
         diags = [Diag() for _ in jobs] if diag else None
         for i, jb in enumerate(jobs):
             pg = jb["grid"]
-            items[i] = SolveItem(pg.handle.value, _dptr(jb["rho"]).value, _dptr(jb["phi"]).value,
-                                 _dptr(jb.get("coef")).value if jb.get("coef") is not None else None,
+            if pg.ctx.device != self.device:
+                raise ValueError(f"job {i}: grid prepared on cuda:{pg.ctx.device}, batch on cuda:{self.device}")
+            items[i] = SolveItem(pg.handle.value, pg._f(jb["rho"], "rho").value, pg._f(jb["phi"], "phi").value,
+                                 pg._c(jb.get("coef")).value if jb.get("coef") is not None else None,
                                  jb.get("coef_mode", COEF_NONE), jb.get("maxtrm", 10),
                                  jb.get("isym", pg.isym), jb.get("npoint", 8), jb.get("redge", 0.0),
                                  C.pointer(diags[i]) if diag else None)
@@ -204,8 +221,10 @@ class Context:
                 raise ValueError(f"{name}: shape {tuple(x.shape)}, expected {shape}")
         if r.numel() < desc.pot_jmax + 2:
             raise ValueError("r: fewer than POT3JMAX+2 values")
-        _check(lib().chy_source_grav(self.handle, C.byref(desc), _dptr(r), _dptr(rho), _dptr(omega), _dptr(phi),
-                                     _dptr(p), _dptr(s), _dptr(t), _dptr(a)), "chy_source_grav")
+        ptr = [_dptr(x, device=self.device, name=n)
+               for n, x in (("r", r), ("rho", rho), ("omega", omega), ("phi", phi), ("p", p), ("s", s), ("t", t),
+                            ("a", a))]
+        _check(lib().chy_source_grav(self.handle, C.byref(desc), *ptr), "chy_source_grav")
 
     def external_pot(self, desc: ExtPotDesc, rhf, zhf, phi, starphi):
         """The star potential of pot/ExternalPot.f (INDIRECT=0, BINARY=0), asynchronous on
@@ -217,8 +236,9 @@ class Context:
             raise ValueError(f"starphi: shape {tuple(starphi.shape)}")
         if rhf.numel() < desc.jmax + 2 or zhf.numel() < desc.kmax + 2:
             raise ValueError("rhf/zhf: too few values")
-        _check(lib().chy_external_pot(self.handle, C.byref(desc), _dptr(rhf), _dptr(zhf), _dptr(phi),
-                                      _dptr(starphi)), "chy_external_pot")
+        ptr = [_dptr(x, device=self.device, name=n)
+               for n, x in (("rhf", rhf), ("zhf", zhf), ("phi", phi), ("starphi", starphi))]
+        _check(lib().chy_external_pot(self.handle, C.byref(desc), *ptr), "chy_external_pot")
 
     def find_force(self, desc: ForceDesc, rhf, zhf, rho, x: float, y: float):
         """find_force of pot/wiggle.F: the disk's pull (fx, fy) on a star at (x, y, 0);
@@ -228,8 +248,8 @@ class Context:
         if rhf.numel() < desc.jmax + 2 or zhf.numel() < desc.kmax + 2:
             raise ValueError("rhf/zhf: too few values")
         f = (C.c_double * 2)()
-        _check(lib().chy_find_force(self.handle, C.byref(desc), _dptr(rhf), _dptr(zhf), _dptr(rho), float(x),
-                                    float(y), f), "chy_find_force")
+        ptr = [_dptr(x, device=self.device, name=n) for n, x in (("rhf", rhf), ("zhf", zhf), ("rho", rho))]
+        _check(lib().chy_find_force(self.handle, C.byref(desc), *ptr, float(x), float(y), f), "chy_find_force")
         return float(f[0]), float(f[1])
 
     def stage_times(self):
@@ -273,24 +293,60 @@ class PreparedGrid:
     def device_bytes(self) -> int:
         return int(lib().chy_grid_device_bytes(self.handle))
 
+    @property
+    def mode_solver(self) -> int:
+        """MODES_CR (BLKTRI's cyclic reduction, the reference's results) or
+        MODES_EIGEN (z eigenbasis on the matrix cores)."""
+        return int(lib().chy_grid_mode_solver(self.handle))
+
+    @mode_solver.setter
+    def mode_solver(self, kind: int):
+        _check(lib().chy_grid_set_mode_solver(self.handle, int(kind)), "chy_grid_set_mode_solver")
+
+    @property
+    def cr_lanes(self) -> int:
+        """Lanes per r system in the cyclic reduction (64 or 32)."""
+        return int(lib().chy_grid_tune(self.handle, TUNE_CR_LANES))
+
+    @cr_lanes.setter
+    def cr_lanes(self, lanes: int):
+        _check(lib().chy_grid_set_tune(self.handle, TUNE_CR_LANES, int(lanes)), "chy_grid_set_tune")
+
+    # device tensor shapes: PHI/RHO (LMAX, KMAX+2, JMAX+2); COEF (2, LMAX/2, KMAX+2, JMAX+2)
+    @property
+    def field_shape(self):
+        g = self.grid
+        return (g.lmax, g.kmax + 2, g.jmax + 2)
+
+    @property
+    def coef_shape(self):
+        g = self.grid
+        return (2, g.lmax // 2, g.kmax + 2, g.jmax + 2)
+
+    def _f(self, t, name):
+        return _dptr(t, self.field_shape, self.ctx.device, name)
+
+    def _c(self, t):
+        return _dptr(t, self.coef_shape, self.ctx.device, "coef")
+
     # -- device-resident forms (torch CUDA float64 tensors) --
     def bdygen(self, rho, phi, maxtrm=10, isym=None, redge=0.0, diag=False):
         d = Diag() if diag else None
         _check(lib().chy_bdygen(self.handle, maxtrm, self.isym if isym is None else isym, redge,
-                                _dptr(rho), _dptr(phi), C.byref(d) if d else None), "chy_bdygen")
+                                self._f(rho, "rho"), self._f(phi, "phi"), C.byref(d) if d else None), "chy_bdygen")
         return d
 
     def pot3(self, rho, phi, npoint=8, coef=None, coef_mode=COEF_NONE, diag=None):
         d = diag if isinstance(diag, Diag) else (Diag() if diag else None)
-        _check(lib().chy_pot3(self.handle, npoint, _dptr(rho), _dptr(phi), _dptr(coef), coef_mode,
-                              C.byref(d) if d else None), "chy_pot3")
+        _check(lib().chy_pot3(self.handle, npoint, self._f(rho, "rho"), self._f(phi, "phi"), self._c(coef),
+                              coef_mode, C.byref(d) if d else None), "chy_pot3")
         return d
 
     def solve(self, rho, phi, maxtrm=10, isym=None, redge=0.0, npoint=8, coef=None,
               coef_mode=COEF_NONE, diag=None):
         d = diag if isinstance(diag, Diag) else (Diag() if diag else None)
         _check(lib().chy_solve(self.handle, maxtrm, self.isym if isym is None else isym, redge, npoint,
-                               _dptr(rho), _dptr(phi), _dptr(coef), coef_mode,
+                               self._f(rho, "rho"), self._f(phi, "phi"), self._c(coef), coef_mode,
                                C.byref(d) if d else None), "chy_solve")
         return d
 
@@ -301,14 +357,14 @@ class PreparedGrid:
         d = diag if isinstance(diag, Diag) else (Diag() if diag else None)
         rel = (C.c_double * steps)() if report else None
         _check(lib().chy_solve_refined(self.handle, maxtrm, self.isym if isym is None else isym, redge, npoint,
-                                       _dptr(rho), _dptr(phi), _dptr(coef), coef_mode, steps,
+                                       self._f(rho, "rho"), self._f(phi, "phi"), self._c(coef), coef_mode, steps,
                                        C.byref(d) if d else None, rel), "chy_solve_refined")
         return d, (list(rel) if report else None)
 
     def zaxphi(self, phi, npoint=8, klocat=0):
         """ZAXPHI alone on a device PHI (pot3.f:311-399); returns the diagnostics."""
         d = Diag(0.0, 0.0, klocat)
-        _check(lib().chy_zaxphi(self.handle, npoint, _dptr(phi), C.byref(d)), "chy_zaxphi")
+        _check(lib().chy_zaxphi(self.handle, npoint, self._f(phi, "phi"), C.byref(d)), "chy_zaxphi")
         return d
 
     def phac(self):
@@ -320,7 +376,7 @@ class PreparedGrid:
 
     # -- host forms (numpy Fortran-ordered float64 arrays, updated in place) --
     def bdygen_host(self, rho, phi, maxtrm=10, isym=None, redge=0.0):
-        _host_ok(rho, phi)
+        _host_ok(self.grid.shape, rho=rho, phi=phi)
         d = Diag()
         dp = C.POINTER(C.c_double)
         _check(lib().chy_bdygen_host(self.handle, maxtrm, self.isym if isym is None else isym, redge,
@@ -329,7 +385,10 @@ class PreparedGrid:
         return d
 
     def pot3_host(self, rho, phi, npoint=8, coef=None, coef_mode=COEF_NONE, klocat=0):
-        _host_ok(rho, phi)
+        _host_ok(self.grid.shape, rho=rho, phi=phi)
+        if coef is not None:
+            g = self.grid
+            _host_ok((g.jmax + 2, g.kmax + 2, g.lmax // 2, 2), coef=coef)
         d = Diag(0.0, 0.0, klocat)
         dp = C.POINTER(C.c_double)
         cp = coef.ctypes.data_as(dp) if coef is not None else None
@@ -349,10 +408,14 @@ class PreparedGrid:
             pass
 
 
-def _host_ok(*arrs):
-    for a in arrs:
+def _host_ok(shape, **arrs):
+    """Host arrays must be Fortran-ordered float64 numpy arrays of `shape`
+    (the COMMON block layout the library copies to and from)."""
+    for name, a in arrs.items():
         if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.f_contiguous):
-            raise ValueError("host arrays must be Fortran-ordered float64 numpy arrays")
+            raise ValueError(f"{name}: host arrays must be Fortran-ordered float64 numpy arrays")
+        if a.shape != tuple(shape):
+            raise ValueError(f"{name}: shape {a.shape}, expected {tuple(shape)}")
 
 
 def to_device(a, device="cuda"):

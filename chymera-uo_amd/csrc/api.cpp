@@ -59,7 +59,7 @@ static bool same_tables(const chy_grid *a, const chy_grid *b)
 {
     return a->J == b->J && a->K == b->K && a->L == b->L && a->isym == b->isym && a->dtheta == b->dtheta &&
            a->pi == b->pi && a->grav == b->grav && a->ncu == b->ncu && a->r == b->r && a->z == b->z &&
-           a->rhf == b->rhf && a->zhf == b->zhf;
+           a->rhf == b->rhf && a->zhf == b->zhf && a->mode_solver == b->mode_solver;
 }
 
 static void free_grid_memory(chy_grid *g)
@@ -718,6 +718,36 @@ int chy_setbdy(chy_ctx *ctx, const chy_grid_desc *desc, int isym, chy_grid **out
     }
     *out = g;
     return CHY_OK;
+}
+
+int chy_grid_set_mode_solver(chy_grid *g, int kind)
+{
+    if (!g || (kind != CHY_MODES_CR && kind != CHY_MODES_EIGEN)) return CHY_EINVAL;
+    if (kind == CHY_MODES_CR && !cr_supported(g)) return CHY_EUNSUPPORTED;
+    if (kind == g->mode_solver) return CHY_OK;
+    for (auto &e : g->graphs) retire_graph(e);
+    g->graphs.clear();
+    g->mode_solver = kind;
+    return CHY_OK;
+}
+
+int chy_grid_mode_solver(const chy_grid *g) { return g ? g->mode_solver : -1; }
+
+int chy_grid_set_tune(chy_grid *g, int param, int value)
+{
+    if (!g) return CHY_EINVAL;
+    if (param != CHY_TUNE_CR_LANES || !cr_lanes_ok(g, value)) return CHY_EINVAL;
+    if (g->cr_lanes == value) return CHY_OK;
+    for (auto &e : g->graphs) retire_graph(e);
+    g->graphs.clear();
+    g->cr_lanes = value;
+    return CHY_OK;
+}
+
+int chy_grid_tune(const chy_grid *g, int param)
+{
+    if (!g || param != CHY_TUNE_CR_LANES) return -1;
+    return g->cr_lanes;
 }
 
 int chy_grid_destroy(chy_grid *g)

@@ -41,6 +41,46 @@ struct Term {
 };
 const Term *terms_table(int *count);  // the kMaxTerms terms, l-major, m ascending
 
+// Cyclic reduction of the per-mode systems (pot/blktri.f:60-260).  A program
+// is one PRDCT product, in one of two forms:
+//   chain (has_pre = 1, ops CR_SCALE): w = sgn x, y = pre w, then for each op
+//     solve (T_m - rt I) w = y along r and y = f w; the result is y;
+//   partial fractions (ops CR_PF): the result is sum over ops of
+//     f (T_m - rt I)^-1 (sgn x), the solves independent of each other.
+enum { CR_SCALE = 2, CR_PF = 3 };
+struct CrOp {
+    double rt, f;
+    int post, pad;
+};
+struct CrProg {
+    int off, nsolve, has_pre, pad;
+    double sgn, pre;
+};
+// One row a phase updates (1-based z row) and its programs (-1: none):
+//   CR_PHASE_ODD  (reduction, rows at odd multiples of h): P(row) = pm(Y(row))
+//   CR_PHASE_EVEN (reduction, rows at even multiples of h):
+//                 Y(row) = (pa(P(row-h)) + pc(P(row+h))) - Y(row)
+//   CR_PHASE_BACK (back substitution, odd multiples of h):
+//                 Y(row) = pm((Y(row) - pa(Y(row-h))) - pc(Y(row+h)))
+enum { CR_PHASE_ODD = 0, CR_PHASE_EVEN = 1, CR_PHASE_BACK = 2 };
+struct CrRow {
+    int row, pa, pc, pm;
+};
+struct CrPhase {
+    int kind, h, row0, nrow;
+};
+struct CrPlan {
+    int kx = 0;
+    long nsolve = 0;               // r solves per mode
+    std::vector<CrPhase> phases;   // in execution order
+    std::vector<CrRow> rows;
+    std::vector<CrProg> progs;
+    std::vector<CrOp> ops;
+};
+// Root tables and programs of the z operator tridiag(an, bn, cn) of order n
+// (an[0] = cn[n-1] = 0), n = 2^k - 1.
+int build_cr_plan(CrPlan &plan, int n, const double *an, const double *bn, const double *cn);
+
 // Device-side grid tables and workspace.  All pointers are device memory.
 struct DeviceGrid {
     // FFT
@@ -61,6 +101,10 @@ struct DeviceGrid {
     double *rc_c = nullptr;      // [M]
     double *rz_an = nullptr;     // [NZ]
     double *rz_cn = nullptr;     // [NZ]
+    // cyclic reduction programs (CrPlan, uploaded)
+    CrOp *cr_ops = nullptr;
+    CrProg *cr_progs = nullptr;
+    CrRow *cr_rows = nullptr;
     // refinement buffers (allocated on first use)
     float *res32 = nullptr;      // residual, PHI layout
     float *spec32 = nullptr;     // [L][KMAX][JMAX]
@@ -174,6 +218,9 @@ struct chy_grid {
     int nring = 0, nchunk = 0, jkmax = 0;
     int ncu = 1;                 // compute units of the device (persistent launches)
     std::vector<double> r, z, rhf, zhf;
+    chy::CrPlan cr;              // per-mode cyclic reduction (host copy of the phases)
+    int mode_solver = CHY_MODES_CR;  // CHY_MODES_CR or CHY_MODES_EIGEN
+    int cr_lanes = 64;               // lanes per r system in the cyclic reduction (64 or 32)
     size_t device_bytes = 0;
     std::vector<void *> allocations;
     chy::DeviceGrid d;
@@ -247,6 +294,12 @@ int launch_fft_inverse32(const chy_grid *g, const float *spec32, double *phi, bo
 // Mode solves on d.spec; rhs_edges applies the boundary terms of Y first
 // (skipped when the spectral BDYGEN has applied them).
 int launch_mode_solve(const chy_grid *g, bool rhs_edges, hipStream_t s, Profiler *prof);
+// BLKTRI's cyclic reduction on every field's spectrum (cr.hip), in place, the
+// boundary terms of Y already applied; cr_supported(g): the grid's r solves
+// fit the kernel (JP/64 in {1, 2, 4, 8}, z rows 2^k - 1).
+bool cr_supported(const chy_grid *g);
+bool cr_lanes_ok(const chy_grid *g, int lanes);
+int launch_cr_solve_f(const chy_grid *g, const Fields &F, hipStream_t s);
 int launch_mode_solve_f(const chy_grid *g, const Fields &F, bool rhs_edges, hipStream_t s, Profiler *prof);
 // The same on binary32 spectrum/eigen-coordinate buffers (refinement
 // corrections: zero boundary values, so no boundary terms).

@@ -556,6 +556,16 @@ int launch_rsolve(const chy_grid *g, const ModeArgs &a, const Fields &F, hipStre
 int launch_mode_solve_f(const chy_grid *g, const Fields &F, bool rhs_edges, hipStream_t s, Profiler *prof)
 {
     ModeArgs a = make_args(g);
+    if (g->mode_solver == CHY_MODES_CR) {
+        if (prof) prof->begin(CHY_STAGE_CR, s);
+        if (rhs_edges) {
+            const long n = (long)(g->M + g->NZ - 1) * g->L;
+            hipLaunchKernelGGL(k_rhs_edges, dim3((unsigned)((n + 255) / 256), F.nf), dim3(256), 0, s, a, F);
+        }
+        int rc = launch_cr_solve_f(g, F, s);
+        if (prof) prof->end(s);
+        return rc ? rc : (int)hipGetLastError();
+    }
     if (prof) prof->begin(CHY_STAGE_ZFWD, s);
     if (rhs_edges) {
         const long n = (long)(g->M + g->NZ - 1) * g->L;

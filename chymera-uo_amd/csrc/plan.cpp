@@ -324,6 +324,10 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
             wi[(size_t)p * PP + k] = (double)(qv / dsc[k]);  // stored [p][k]: reduction index major
         }
 
+    // ---- BLKTRI's root tables and the cyclic reduction's programs (crplan.cpp) ----
+    CHY_TRY(build_cr_plan(g->cr, NZ, an.data(), bn.data(), cn.data()));
+    g->mode_solver = cr_supported(g) ? CHY_MODES_CR : CHY_MODES_EIGEN;
+
     // ---- FFT twiddles (exact angles; the reference uses recurrences, fft.f:225-232) ----
     std::vector<double2> twL(L), tw64(64);
     for (int e = 0; e < L; ++e) {
@@ -449,6 +453,9 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
         CHY_TRY(upload(g, &d.bm, bmp));
     }
     CHY_TRY(upload(g, &d.zx, zx));
+    CHY_TRY(upload(g, &d.cr_ops, g->cr.ops));
+    CHY_TRY(upload(g, &d.cr_progs, g->cr.progs));
+    CHY_TRY(upload(g, &d.cr_rows, g->cr.rows));
     // the 7-point operator of the same system for refinement residuals
     CHY_TRY(upload(g, &d.rc_c, c));
     CHY_TRY(upload(g, &d.rz_an, an));

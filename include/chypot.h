@@ -100,6 +100,35 @@ int chy_grid_destroy(chy_grid *g);
 /* Device bytes held by a prepared grid (tables + workspace). */
 size_t chy_grid_device_bytes(const chy_grid *g);
 
+/* How POT3's per-mode 2-D solves (pot/pot3.f:233-239) are done on a grid.
+ *   CHY_MODES_CR    (default) BLKTRI's generalised cyclic reduction
+ *                   (pot/blktri.f:60-260) with BLKTRI's own root tables
+ *                   (COMPB/TQLRAT, pot/blktri.f:264-423, the same arithmetic):
+ *                   the reference's results to rounding (~1e-14 relative).
+ *   CHY_MODES_EIGEN the m-independent z operator diagonalised once per grid
+ *                   (80-bit arithmetic) and r-tridiagonal solves per z
+ *                   eigenvalue, the transforms on FP64 matrix cores: closer to
+ *                   the exact solution of the discrete system than BLKTRI
+ *                   (whose root tables put it up to a few 1e-12 from it at
+ *                   256x128x256), and faster.
+ * CHY_MODES_CR needs JMAX-1 <= 512 (CHY_EUNSUPPORTED otherwise; such grids
+ * start with CHY_MODES_EIGEN).  Changing the solver drops the grid's captured
+ * solve graphs. */
+enum { CHY_MODES_CR = 0, CHY_MODES_EIGEN = 1 };
+int chy_grid_set_mode_solver(chy_grid *g, int kind);
+int chy_grid_mode_solver(const chy_grid *g);
+
+/* Kernel shapes of a grid (no reference counterpart: launch configuration).
+ *   CHY_TUNE_CR_LANES  lanes per r system in the cyclic reduction: 64 (one
+ *                      order per wavefront) or 32 (two orders per wavefront,
+ *                      JMAX-1 <= 256)
+ * Results do not depend on the shapes beyond rounding.  Changing a shape
+ * drops the grid's captured solve graphs.  CHY_EINVAL for a value the grid
+ * cannot take. */
+enum { CHY_TUNE_CR_LANES = 0 };
+int chy_grid_set_tune(chy_grid *g, int param, int value);
+int chy_grid_tune(const chy_grid *g, int param);
+
 /* ---- BDYGEN (pot/boundary.f:88-908) ----
  * Multipole boundary potential (l <= maxtrm <= 10) from rho_d into the boundary
  * cells of phi_d: K=KMAX+1 for J=2..JMAX+1, J=JMAX+1 for K=2..KMAX, and the
@@ -274,6 +303,7 @@ enum {
     CHY_STAGE_FFT_INV,    /* inverse azimuthal RFFT */
     CHY_STAGE_GHOST,      /* ghosts, ZAXPHI, PHICHK */
     CHY_STAGE_FFT_BDY,    /* forward RFFT of the boundary lanes (chy_solve) */
+    CHY_STAGE_CR,         /* per-mode cyclic reduction (CHY_MODES_CR) */
     CHY_NSTAGES
 };
 int chy_ctx_profile(chy_ctx *ctx, int enable);

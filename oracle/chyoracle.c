@@ -599,6 +599,46 @@ static void cr_product(int m, const double *am, const double *bm, const double *
     }
 }
 
+/* The product of a multiplier program (na = 0) by partial fractions: with
+ * divisor roots b_i and multiplier roots c_j (one fewer), prod(T - c_j) /
+ * prod(T - b_i) = sum_i alpha_i (T - b_i)^-1, alpha_i = prod_j (b_i - c_j) /
+ * prod_{k != i} (b_i - b_k).  BLKTRI's multipliers are the eigenvalues of the
+ * two halves of the divisor's block, which interlace with the divisor roots
+ * (Cauchy), so every alpha_i >= 0 and the sum has no cancellation.  The
+ * solves are independent (the form a parallel machine takes).  Variant 2 of
+ * cr_rsolve_kind's companion switch cr_pf. */
+static int cr_pf = 0;
+
+void ora_cr_partial_fractions(int on) { cr_pf = on; }
+
+static void cr_product_pf(int m, const double *am, const double *bm, const double *cm, int nd,
+                          const double *div, int n1, const double *mult1, int n2, const double *mult2,
+                          const double *x, double *y, double *w, double *work, int sgn)
+{
+    long double *c = malloc(sizeof(long double) * (n1 + n2 + 1));
+    int nc = 0;
+    for (int j = 0; j < n1; ++j) c[nc++] = mult1[j];
+    for (int j = 0; j < n2; ++j) c[nc++] = mult2[j];
+    double *xs = malloc(sizeof(double) * m);
+    for (int r = 0; r < m; ++r) { xs[r] = (nd > 0 && sgn <= 0) ? -x[r] : x[r]; y[r] = 0; }
+    for (int i = nd - 1; i >= 0; --i) {
+        /* alpha_i, each numerator factor paired with a denominator factor */
+        long double al = 1.0L;
+        int kk = 0;
+        for (int j = 0; j < nc; ++j) {
+            while (kk == i) ++kk;
+            al *= ((long double)div[i] - c[j]) / ((long double)div[i] - div[kk]);
+            ++kk;
+        }
+        for (; kk < nd; ++kk)
+            if (kk != i) al /= ((long double)div[i] - div[kk]);
+        cr_rsolve(m, am, bm, cm, div[i], xs, w, work);
+        double a = (double)al;
+        for (int r = 0; r < m; ++r) y[r] += a * w[r];
+    }
+    free(c); free(xs);
+}
+
 /* y is Y(M,N) column-major (r fastest), overwritten by X.  cnp is CN shifted
  * one row down (cnp[i] = cn[i-1], cnp[0] = cn[n-1] = 0), the form BLKTRI
  * hands its reduction (pot/blktri.f:46-51). */
@@ -617,8 +657,12 @@ static void cr_solve(const cr_roots *t, int m, const double *an, const double *c
         int sg = (lv & 1) ? -1 : 1;
         for (int o = h; o <= n; o += 2 * h) {
             int b = o / h - 1;
-            cr_product(m, am, bm, cm, iiz, cr_blk(t, lv, b), iim, lv ? cr_blk(t, lv - 1, 2 * b) : NULL,
-                       iim, lv ? cr_blk(t, lv - 1, 2 * b + 2) : NULL, 0, NULL, YC(o), PC(o), ww, wk, sg);
+            if (cr_pf)
+                cr_product_pf(m, am, bm, cm, iiz, cr_blk(t, lv, b), iim, lv ? cr_blk(t, lv - 1, 2 * b) : NULL,
+                              iim, lv ? cr_blk(t, lv - 1, 2 * b + 2) : NULL, YC(o), PC(o), ww, wk, sg);
+            else
+                cr_product(m, am, bm, cm, iiz, cr_blk(t, lv, b), iim, lv ? cr_blk(t, lv - 1, 2 * b) : NULL,
+                           iim, lv ? cr_blk(t, lv - 1, 2 * b + 2) : NULL, 0, NULL, YC(o), PC(o), ww, wk, sg);
         }
         for (int e = 2 * h; e + h <= n; e += 2 * h) {
             int s = e / (2 * h);
@@ -648,9 +692,14 @@ static void cr_solve(const cr_roots *t, int m, const double *an, const double *c
                 for (int r = 0; r < m; ++r) w2[r] = 0;
             double *yi = YC(i);
             for (int r = 0; r < m; ++r) w1[r] = yi[r] - w1[r] - w2[r];
-            cr_product(m, am, bm, cm, iiz, cr_blk(t, lv, jj - 1), iim,
-                       lv ? cr_blk(t, lv - 1, 2 * jj - 2) : NULL, iim, lv ? cr_blk(t, lv - 1, 2 * jj) : NULL,
-                       0, NULL, w1, yi, ww, wk, sg);
+            if (cr_pf)
+                cr_product_pf(m, am, bm, cm, iiz, cr_blk(t, lv, jj - 1), iim,
+                              lv ? cr_blk(t, lv - 1, 2 * jj - 2) : NULL, iim, lv ? cr_blk(t, lv - 1, 2 * jj) : NULL,
+                              w1, yi, ww, wk, sg);
+            else
+                cr_product(m, am, bm, cm, iiz, cr_blk(t, lv, jj - 1), iim,
+                           lv ? cr_blk(t, lv - 1, 2 * jj - 2) : NULL, iim, lv ? cr_blk(t, lv - 1, 2 * jj) : NULL,
+                           0, NULL, w1, yi, ww, wk, sg);
         }
     }
 #undef YC

@@ -80,7 +80,43 @@ def test_random_field_draws_documented_ranges():
     assert len(seen) > 3
 
 
-def test_host_arrays_must_be_fortran_float64():
+def test_host_arrays_must_be_fortran_float64_of_the_grid_shape():
     with pytest.raises(ValueError):
-        chypot._host_ok(np.zeros((3, 3, 3)))
-    chypot._host_ok(np.zeros((3, 3, 3), order="F"))
+        chypot._host_ok((3, 3, 3), rho=np.zeros((3, 3, 3)))
+    with pytest.raises(ValueError, match="shape"):
+        chypot._host_ok((3, 3, 4), phi=np.zeros((3, 3, 3), order="F"))
+    with pytest.raises(ValueError):
+        chypot._host_ok((3, 3, 3), coef=np.zeros((3, 3, 3), dtype=np.float32, order="F"))
+    chypot._host_ok((3, 3, 3), rho=np.zeros((3, 3, 3), order="F"))
+
+
+class _FakeCudaTensor:
+    """Stands in for a torch CUDA tensor in the argument checks (no GPU here)."""
+
+    def __init__(self, shape, device=0, dtype="torch.float64", contiguous=True):
+        self.shape, self.is_cuda = tuple(shape), True
+        self.dtype = type("D", (), {"__repr__": lambda _s: dtype})()
+        self.device = type("Dev", (), {"index": device})()
+        self._c = contiguous
+
+    def is_contiguous(self):
+        return self._c
+
+    def data_ptr(self):
+        return 4096
+
+
+def test_device_arguments_are_checked_against_grid_and_device():
+    shape = (16, 10, 18)
+    assert chypot._dptr(_FakeCudaTensor(shape), shape, 0, "phi").value == 4096
+    with pytest.raises(ValueError, match="shape"):
+        chypot._dptr(_FakeCudaTensor((16, 10, 17)), shape, 0, "phi")
+    with pytest.raises(ValueError, match="cuda:1"):
+        chypot._dptr(_FakeCudaTensor(shape, device=1), shape, 0, "rho")
+    with pytest.raises(ValueError, match="float64"):
+        chypot._dptr(_FakeCudaTensor(shape, dtype="torch.float32"), shape, 0, "rho")
+    with pytest.raises(ValueError):
+        chypot._dptr(_FakeCudaTensor(shape, contiguous=False), shape, 0, "rho")
+    # COEF(JMAX+2, KMAX+2, LMAX/2, 2) in Fortran order is a (2, LMAX/2, KMAX+2, JMAX+2) tensor
+    with pytest.raises(ValueError, match="shape"):
+        chypot._dptr(_FakeCudaTensor((18, 10, 8, 2)), (2, 8, 10, 18), 0, "coef")
