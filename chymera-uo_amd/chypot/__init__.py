@@ -23,7 +23,7 @@ from . import fields  # noqa: F401  (re-exported helper module)
 _HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = _HERE.parent / "_lib" / "libchypot.so"
 
-CHY_OK, CHY_EINVAL, CHY_EUNSUPPORTED, CHY_ENOMEM, CHY_EDEVICE, CHY_ESTATE, CHY_ENUMERIC = range(7)
+CHY_OK, CHY_EINVAL, CHY_EUNSUPPORTED, CHY_ENOMEM, CHY_EDEVICE, CHY_ESTATE, CHY_ENUMERIC, CHY_ELIMIT = range(8)
 COEF_NONE, COEF_K2, COEF_FULL = 0, 1, 2
 STAGES = ("bdygen", "fft_fwd", "zfwd", "rsolve", "zinv", "fft_inv", "ghost", "fft_bdy", "cr")
 # per-mode solver of a grid (chy_grid_set_mode_solver)
@@ -39,7 +39,7 @@ EXPORTS = (
     "chy_bdygen", "chy_pot3", "chy_solve", "chy_bdygen_host", "chy_pot3_host",
     "chy_ctx_profile", "chy_ctx_stage_times", "chy_ctx_graphs",
     "chy_zaxphi", "chy_zaxphi_host", "chy_phac", "chy_solve_batch", "chy_ctx_workers",
-    "chy_solve_refined", "chy_source_grav", "chy_external_pot", "chy_find_force",
+    "chy_solve_refined", "chy_source_grav", "chy_external_pot", "chy_find_force", "chy_find_force_noalloc",
     "chy_strerror", "chy_last_device_error",
 )
 
@@ -131,6 +131,7 @@ def lib():
         L.chy_source_grav.argtypes = [vp, C.POINTER(SourceDesc)] + [vp] * 8
         L.chy_external_pot.argtypes = [vp, C.POINTER(ExtPotDesc)] + [vp] * 4
         L.chy_find_force.argtypes = [vp, C.POINTER(ForceDesc), vp, vp, vp, C.c_double, C.c_double, dp]
+        L.chy_find_force_noalloc.argtypes = [vp, C.POINTER(ForceDesc), vp, vp, vp, C.c_double, C.c_double, vp]
         L.chy_strerror.argtypes = [C.c_int]
         L.chy_strerror.restype = C.c_char_p
         L.chy_last_device_error.restype = C.c_char_p
@@ -251,6 +252,18 @@ class Context:
         ptr = [_dptr(x, device=self.device, name=n) for n, x in (("rhf", rhf), ("zhf", zhf), ("rho", rho))]
         _check(lib().chy_find_force(self.handle, C.byref(desc), *ptr, float(x), float(y), f), "chy_find_force")
         return float(f[0]), float(f[1])
+
+    def find_force_noalloc(self, desc: ForceDesc, rhf, zhf, rho, x: float, y: float, f):
+        """find_force into the device tensor f (2 float64: fx, fy) without any
+        allocation (the library's fixed per-GPU scratch); asynchronous on this
+        context's stream."""
+        if tuple(rho.shape) != (desc.lmax, desc.kmax + 2, desc.jmax + 2):
+            raise ValueError(f"rho: shape {tuple(rho.shape)}")
+        if rhf.numel() < desc.jmax + 2 or zhf.numel() < desc.kmax + 2:
+            raise ValueError("rhf/zhf: too few values")
+        ptr = [_dptr(v, device=self.device, name=n) for n, v in (("rhf", rhf), ("zhf", zhf), ("rho", rho))]
+        _check(lib().chy_find_force_noalloc(self.handle, C.byref(desc), *ptr, float(x), float(y),
+                                            _dptr(f, (2,), self.device, "f")), "chy_find_force_noalloc")
 
     def stage_times(self):
         """{stage: (total_ms, launches)} since profiling was enabled (waits)."""

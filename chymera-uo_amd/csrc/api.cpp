@@ -462,6 +462,7 @@ const char *chy_strerror(int code)
     case CHY_EDEVICE: return "HIP runtime error";
     case CHY_ESTATE: return "call out of order";
     case CHY_ENUMERIC: return "z operator not symmetrisable or eigen-solver did not converge";
+    case CHY_ELIMIT: return "input larger than the fixed scratch of a no-allocation entry point";
     default: return "unknown error";
     }
 }
@@ -485,6 +486,13 @@ int chy_ctx_create(int device, chy_ctx **out)
         return CHY_EDEVICE;
     }
     c->stream = c->own;
+    e = (hipError_t)force_scratch_address(device, &c->force_scratch);
+    if (e != hipSuccess) {
+        set_device_error(e, "hipGetSymbolAddress");
+        (void)hipStreamDestroy(c->own);
+        delete c;
+        return CHY_EDEVICE;
+    }
     *out = c;
     return CHY_OK;
 }
@@ -1001,6 +1009,25 @@ int chy_find_force(chy_ctx *ctx, const chy_force_desc *d, const double *rhf_d, c
         f_h[1] = ctx->force_h[1];
     }
     return rc;
+}
+
+int chy_find_force_noalloc(chy_ctx *ctx, const chy_force_desc *d, const double *rhf_d, const double *zhf_d,
+                           const double *rho_d, double x, double y, double *f_d)
+{
+    if (!ctx) return CHY_ESTATE;
+    if (!d || !rhf_d || !zhf_d || !rho_d || !f_d) return CHY_EINVAL;
+    if (d->jmax < 1 || d->kmax < 1 || d->lmax < 1 || d->jmin < 4 || d->jmin > d->jmax + 1) return CHY_EINVAL;
+    if (!(d->dr > 0.0) || !(d->dz > 0.0) || !(d->dphi > 0.0)) return CHY_EINVAL;
+    const long nb = (long)force_blocks(d);
+    if (nb > kForceNoAllocBlocks || d->lmax > kForceNoAllocL) {
+        g_last_error = "chy_find_force_noalloc: " + std::to_string(nb) + " block partials and LMAX " +
+                       std::to_string(d->lmax) + " exceed its scratch (" + std::to_string(kForceNoAllocBlocks) +
+                       " partials, LMAX " + std::to_string(kForceNoAllocL) + ")";
+        return CHY_ELIMIT;
+    }
+    DeviceScope ds(ctx->device);
+    return dev_rc(launch_find_force_noalloc(d, rhf_d, zhf_d, rho_d, x, y, ctx->force_scratch, f_d, ctx->stream),
+                  "find_force launch");
 }
 
 int chy_external_pot(chy_ctx *ctx, const chy_extpot_desc *d, const double *rhf_d, const double *zhf_d,

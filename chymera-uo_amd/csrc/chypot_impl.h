@@ -186,6 +186,8 @@ struct chy_ctx {
     // chy_find_force: block partials and the (fx, fy) result
     double *force_part = nullptr, *force_out = nullptr, *force_h = nullptr;
     int force_cap = 0;
+    // chy_find_force_noalloc: the device's fixed scratch (force.hip)
+    double *force_scratch = nullptr;
 };
 
 namespace chy {
@@ -332,6 +334,14 @@ int launch_external_pot(const chy_extpot_desc *d, const double *rhf, const doubl
 // find_force of pot/wiggle.F: block partials into part (force_blocks(d) pairs),
 // the total (fx, fy) into out.
 int force_blocks(const chy_force_desc *d);
+// chy_find_force_noalloc: largest partial count and LMAX its fixed scratch
+// holds; the scratch's device address on a device (taken once per device);
+// the launch on that scratch (cos/sin table built on the device).
+constexpr int kForceNoAllocBlocks = 65536;
+constexpr int kForceNoAllocL = 4096;
+int force_scratch_address(int device, double **out);
+int launch_find_force_noalloc(const chy_force_desc *d, const double *rhf, const double *zhf, const double *rho,
+                              double x, double y, double *scr, double *out, hipStream_t st);
 int launch_find_force(const chy_force_desc *d, const double *rhf, const double *zhf, const double *cs,
                       const double *rho, double x, double y, double *part, double *out, hipStream_t st);
 // The momentum sources of radhydro/source.f:41-79 (arguments checked by the caller).

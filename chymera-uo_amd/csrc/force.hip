@@ -22,6 +22,9 @@
 // the sum with J fastest: a wavefront reads 64 consecutive cells of a plane,
 // and no block is left with a few live lanes when the row of NJ = JMAX-JMIN+5
 // cells is not a multiple of 64.  cos/sin of phi_L come from a table.
+#include <mutex>
+#include <vector>
+
 #include "chypot_impl.h"
 
 namespace chy {
@@ -110,7 +113,54 @@ __global__ __launch_bounds__(256) void k_force_total(const double *part, int npa
     }
 }
 
+// cos/sin of phi_L = L DPHI - DPHI/2 (the angle find_force forms,
+// pot/wiggle.F:113) for the no-allocation form, which has no host table
+__global__ __launch_bounds__(256) void k_force_trig(double *cs, int lmax, double dphi)
+{
+    const int l = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (l >= lmax) return;
+    const double a = (double)(l + 1) * dphi - 0.5 * dphi;
+    cs[2 * l] = cos(a);
+    cs[2 * l + 1] = sin(a);
+}
+
+// Scratch of chy_find_force_noalloc: the block partials (two doubles each)
+// followed by the cos/sin table, in device memory the library never
+// allocates at run time.  One copy per GPU, shared by every call on it.
+__device__ __align__(256) double scratch[2 * kForceNoAllocBlocks + 2 * kForceNoAllocL];
+
 }  // namespace
+
+// The scratch array's device address on `device` (host code may not use the
+// array's name as a device pointer).  Taken once per device, with the device
+// current, at the first context created on it.
+int force_scratch_address(int device, double **out)
+{
+    static std::mutex mu;
+    static std::vector<double *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0) return (int)hipErrorInvalidDevice;
+    if ((int)addr.size() <= device) addr.resize((size_t)device + 1, nullptr);
+    if (!addr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        addr[device] = static_cast<double *>(p);
+    }
+    *out = addr[device];
+    return (int)hipSuccess;
+}
+
+int launch_find_force_noalloc(const chy_force_desc *d, const double *rhf, const double *zhf, const double *rho,
+                              double x, double y, double *scr, double *out, hipStream_t st)
+{
+    double *part = scr, *cs = scr + 2 * (size_t)kForceNoAllocBlocks;
+    hipLaunchKernelGGL(k_force_trig, dim3((unsigned)((d->lmax + 255) / 256)), dim3(256), 0, st, cs, d->lmax,
+                       d->dphi);
+    int rc = (int)hipGetLastError();
+    if (rc) return rc;
+    return launch_find_force(d, rhf, zhf, cs, rho, x, y, part, out, st);
+}
 
 static int force_row_blocks(const chy_force_desc *d)
 {

@@ -51,8 +51,11 @@ enum {
     CHY_EDEVICE = 4,      /* HIP runtime error */
     CHY_ESTATE = 5,       /* call out of order (e.g. BDYGEN before SETBDY,
                              pot/boundary.f:311) */
-    CHY_ENUMERIC = 6      /* z operator not symmetrisable (AN(k)*CN(k-1) <= 0) or
+    CHY_ENUMERIC = 6,     /* z operator not symmetrisable (AN(k)*CN(k-1) <= 0) or
                              eigen-solver failed to converge */
+    CHY_ELIMIT = 7        /* input larger than the fixed scratch of a
+                             no-allocation entry point (chy_last_device_error
+                             names the sizes) */
 };
 
 /* Grid as the reference holds it in /GRID/ and /blok6/. */
@@ -288,6 +291,21 @@ typedef struct {
 } chy_force_desc;
 int chy_find_force(chy_ctx *ctx, const chy_force_desc *d, const double *rhf_d, const double *zhf_d,
                    const double *rho_d, double x, double y, double *f_h);
+
+/* find_force without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device or host memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * sum as chy_find_force into f_d (device, 2 doubles: fx, fy), asynchronous on
+ * the context's stream (no wait, no host copy).  Its block partials and
+ * cos/sin table live in a fixed per-GPU scratch array (1.1 MiB, held until
+ * the process ends), so it takes at most 65536 block partials (the partial
+ * count is ceil((JMAX-JMIN+5)*KMAX/256) * ceil(LMAX/16)) and LMAX <= 4096,
+ * else CHY_ELIMIT.  The cos/sin table is computed on the GPU, so results agree
+ * with chy_find_force's to rounding, not bit for bit.  Every call on one GPU
+ * shares that scratch: calls that overlap on one GPU (other streams, other
+ * contexts, other threads) race. */
+int chy_find_force_noalloc(chy_ctx *ctx, const chy_force_desc *d, const double *rhf_d, const double *zhf_d,
+                           const double *rho_d, double x, double y, double *f_d);
 
 /* ---- stage timing (no reference counterpart; for benchmarks) ----
  * While enabled, every stage of chy_bdygen/chy_pot3/chy_solve is bracketed by
