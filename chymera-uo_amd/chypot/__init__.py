@@ -28,8 +28,8 @@ COEF_NONE, COEF_K2, COEF_FULL = 0, 1, 2
 STAGES = ("bdygen", "fft_fwd", "zfwd", "rsolve", "zinv", "fft_inv", "ghost", "fft_bdy", "cr")
 # per-mode solver of a grid (chy_grid_set_mode_solver)
 MODES_CR, MODES_EIGEN = 0, 1
-# tunable kernel shapes (chy_grid_set_tune)
-TUNE_CR_LANES = 0
+# tunable kernel shapes and precision split (chy_grid_set_tune)
+TUNE_CR_LANES, TUNE_REFINE_M64 = 0, 1
 
 # every entry point declared in include/chypot.h
 EXPORTS = (
@@ -38,7 +38,7 @@ EXPORTS = (
     "chy_grid_set_mode_solver", "chy_grid_mode_solver", "chy_grid_set_tune", "chy_grid_tune",
     "chy_bdygen", "chy_pot3", "chy_solve", "chy_bdygen_host", "chy_pot3_host",
     "chy_ctx_profile", "chy_ctx_stage_times", "chy_ctx_graphs",
-    "chy_zaxphi", "chy_zaxphi_host", "chy_phac", "chy_solve_batch", "chy_ctx_workers",
+    "chy_zaxphi", "chy_zaxphi_host", "chy_phac", "chy_solve_batch", "chy_solve_batch_refined", "chy_ctx_workers",
     "chy_solve_refined", "chy_source_grav", "chy_external_pot", "chy_find_force", "chy_find_force_noalloc",
     "chy_strerror", "chy_last_device_error",
 )
@@ -124,6 +124,7 @@ def lib():
         L.chy_zaxphi_host.argtypes = [vp, C.c_int, dp, C.POINTER(Diag)]
         L.chy_phac.argtypes = [vp, dp, C.c_int]
         L.chy_solve_batch.argtypes = [vp, C.POINTER(SolveItem), C.c_int]
+        L.chy_solve_batch_refined.argtypes = [vp, C.POINTER(SolveItem), C.c_int, C.c_int]
         L.chy_ctx_workers.argtypes = [vp, C.c_int]
         L.chy_solve_refined.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_int, vp, vp, vp, C.c_int,
                                         C.c_int, C.POINTER(Diag), dp]
@@ -189,11 +190,12 @@ class Context:
         """Worker streams used by solve_batch (1..16)."""
         _check(lib().chy_ctx_workers(self.handle, int(n)), "chy_ctx_workers")
 
-    def solve_batch(self, jobs, diag=False):
+    def solve_batch(self, jobs, diag=False, refine_steps=0):
         """Solve independent fields in one call.  jobs: iterable of dicts with
         keys grid (PreparedGrid), rho, phi and optionally coef, coef_mode,
-        maxtrm, isym, redge, npoint.  Returns a list of Diag (diag=True, waits)
-        or None (asynchronous on this context's stream)."""
+        maxtrm, isym, redge, npoint.  refine_steps > 0: every field by the
+        refined solve with that many correction steps.  Returns a list of Diag
+        (diag=True, waits) or None (asynchronous on this context's stream)."""
         jobs = list(jobs)
         items = (SolveItem * len(jobs))()
         diags = [Diag() for _ in jobs] if diag else None
@@ -206,7 +208,11 @@ class Context:
                                  jb.get("coef_mode", COEF_NONE), jb.get("maxtrm", 10),
                                  jb.get("isym", pg.isym), jb.get("npoint", 8), jb.get("redge", 0.0),
                                  C.pointer(diags[i]) if diag else None)
-        _check(lib().chy_solve_batch(self.handle, items, len(jobs)), "chy_solve_batch")
+        if refine_steps > 0:
+            _check(lib().chy_solve_batch_refined(self.handle, items, len(jobs), int(refine_steps)),
+                   "chy_solve_batch_refined")
+        else:
+            _check(lib().chy_solve_batch(self.handle, items, len(jobs)), "chy_solve_batch")
         return diags
 
     def source_grav(self, desc: SourceDesc, r, rho, omega, phi, p, s, t, a):
@@ -324,6 +330,15 @@ class PreparedGrid:
     @cr_lanes.setter
     def cr_lanes(self, lanes: int):
         _check(lib().chy_grid_set_tune(self.handle, TUNE_CR_LANES, int(lanes)), "chy_grid_set_tune")
+
+    @property
+    def refine_m64(self) -> int:
+        """Refined solves: orders m <= this value are solved in binary64."""
+        return int(lib().chy_grid_tune(self.handle, TUNE_REFINE_M64))
+
+    @refine_m64.setter
+    def refine_m64(self, m: int):
+        _check(lib().chy_grid_set_tune(self.handle, TUNE_REFINE_M64, int(m)), "chy_grid_set_tune")
 
     # device tensor shapes: PHI/RHO (LMAX, KMAX+2, JMAX+2); COEF (2, LMAX/2, KMAX+2, JMAX+2)
     @property

@@ -42,6 +42,7 @@ void set_field(Fields &F, int f, const chy_grid *w, const void *rho, double *phi
     F.dscal[f] = w->d.dscal;
     F.ticket[f] = w->d.ticket;
     F.zsamp[f] = w->d.zsamp;
+    F.rstat[f] = w->d.rstat;
     F.klocat_in[f] = klocat_in;
 }
 
@@ -358,38 +359,85 @@ static int ensure_refine(chy_grid *g)
         g->device_bytes += bytes;
         return CHY_OK;
     };
-    int rc = get((void **)&g->d.res32, sizeof(float) * field_elems(g));
-    if (!rc) rc = get((void **)&g->d.spec32, sizeof(float) * (size_t)g->L * g->K * g->J);
+    if (!refine_supported(g)) return CHY_EUNSUPPORTED;
+    int rc = get((void **)&g->d.spec32, sizeof(float) * (size_t)g->L * g->K * g->J);
     if (!rc) rc = get((void **)&g->d.yh32, sizeof(float) * (size_t)g->L * g->PP * g->JP);
     if (!rc) rc = get((void **)&g->d.rstat, sizeof(unsigned long long) * 2 * kMaxRefine);
     return rc;
 }
 
-// BDYGEN in binary64 (boundary values into PHI), then `steps` corrections:
-// binary64 residual of the discrete operator, correction solved with binary32
-// spectra, added to PHI in binary64; POT3's ghosts with the last step, ZAXPHI.
+// The right-hand side spectrum of POT3's mode systems for every field of F:
+// the interior lanes' transform (with BDYGEN's ring moments and COEF), then
+// BDYGEN -- in the spectrum when LMAX/2 > MAXTRM's 10, else through PHI, a
+// transform of the boundary lanes and the boundary terms of Y.
+static int rhs_spectrum_f(chy_grid *g, const Fields &F, hipStream_t s, int maxtrm, int isym, double redge,
+                          int coef_mode)
+{
+    const bool spectral = bdygen_spectral_ok(g);
+    int rc = dev_rc(launch_fft_forward_f(g, F, coef_mode, LANES_INTERIOR, true, true, s), "forward FFT launch");
+    if (!rc) rc = dev_rc(launch_bdygen_from_moments_f(g, F, maxtrm, isym, redge, spectral, coef_mode, s),
+                         "bdygen launch");
+    if (!rc && !spectral) {
+        rc = dev_rc(launch_fft_forward_f(g, F, coef_mode, LANES_BOUNDARY, false, true, s), "boundary FFT launch");
+        if (!rc) rc = dev_rc(launch_rhs_edges_f(g, F, s), "boundary terms launch");
+    }
+    return rc;
+}
+
+// Refined solve of every field of F (grids[f] owns field f's workspace; g's
+// tables), by iterative refinement of the mode systems in the azimuthal
+// spectrum, where the 7-point operator is block diagonal by mode:
+//   B = the right-hand side spectrum (binary64, as the plain solve forms it);
+//   X = 0; for each step: R = B - A X (binary64, per mode, rounded to
+//   binary32), D = A^-1 R by the binary32 mode solves (binary64 r solves for
+//   orders m <= CHY_TUNE_REFINE_M64), X += D (binary64);
+//   then the inverse transform of X (binary64) into PHI with POT3's ghosts,
+//   and ZAXPHI.
+// The first step's R is B itself.  X lives in the grid's binary64 eigen
+// coordinate buffer (unused by this path), R and the eigen coordinates in
+// binary32 buffers.  with_stats: per step max |D| and max |X| into d.rstat.
+static int run_refined_fields(chy_grid *g, const std::vector<chy_grid *> &grids, const Fields &F, hipStream_t s,
+                              int maxtrm, int isym, double redge, int npoint, int coef_mode, int steps,
+                              bool with_stats)
+{
+    if (!refine_supported(g)) return CHY_EUNSUPPORTED;
+    int rc = rhs_spectrum_f(g, F, s, maxtrm, isym, redge, coef_mode);
+    if (rc) return rc;
+    if (with_stats) rc = dev_rc(launch_zero_u64_f(F, 2 * kMaxRefine, s), "statistics clear");
+    Fields Fr = F, Ff0 = F, Ff = F, Fi = F;
+    for (int f = 0; f < F.nf; ++f) {
+        const DeviceGrid &d = grids[f]->d;
+        Fr.aux[f] = d.yh;       // residual: B (spec) and X in, R out
+        Fr.yh[f] = d.spec32;
+        Ff0.yh[f] = d.yh32;     // first step: B (binary64) in
+        Ff.spec[f] = d.spec32;  // later steps: R (binary32) in
+        Ff.yh[f] = d.yh32;
+        Fi.yh[f] = d.yh32;      // inverse: into X, the last step into spec
+        Fi.aux[f] = d.yh;
+    }
+    for (int it = 0; it < steps && !rc; ++it) {
+        const bool last = it == steps - 1;
+        const int epi = steps == 1 ? EPI_SPEC : (it == 0 ? EPI_XSET : (last ? EPI_XFIN : EPI_XADD));
+        if (it > 0) rc = dev_rc(launch_spec_residual_f(g, Fr, s), "spectral residual launch");
+        if (!rc)
+            rc = dev_rc(launch_mode_solve32_f(g, it == 0 ? Ff0 : Ff, Fi, g->refine_m64, it == 0, epi,
+                                              with_stats ? it : -1, s),
+                        "correction mode solve launch");
+    }
+    if (!rc) rc = dev_rc(launch_fft_inverse_f(g, F, s), "inverse FFT launch");
+    if (!rc) rc = dev_rc(launch_zaxphi_f(g, F, npoint, zaxphi_from_samples(g), s), "ZAXPHI launch");
+    return rc;
+}
+
 static int run_refined(chy_grid *g, hipStream_t s, int maxtrm, int isym, double redge, int npoint,
-                       const double *rho, double *phi, double *coef, int coef_mode, int steps, int klocat_in)
+                       const double *rho, double *phi, double *coef, int coef_mode, int steps, int klocat_in,
+                       bool with_stats)
 {
     int rc = ensure_refine(g);
     if (rc) return rc;
-    rc = dev_rc(launch_fft_forward(g, rho, phi, coef, coef_mode, LANES_INTERIOR, true, false, s),
-                "moment transform launch");
-    if (!rc) rc = dev_rc(launch_bdygen_from_moments(g, maxtrm, isym, redge, phi, false, nullptr, CHY_COEF_NONE, s),
-                         "bdygen launch");
-    if (!rc && coef_mode != CHY_COEF_NONE)  // COEF of the boundary lanes
-        rc = dev_rc(launch_fft_forward(g, rho, phi, coef, coef_mode, LANES_BOUNDARY, false, false, s),
-                    "boundary COEF launch");
-    for (int it = 0; it < steps && !rc; ++it) {
-        rc = dev_rc(launch_residual(g, rho, phi, it == 0, it == 0, s), "residual launch");
-        if (!rc) rc = dev_rc(launch_fft_forward_res32(g, g->d.res32, g->d.spec32, s), "correction FFT launch");
-        if (!rc) rc = dev_rc(launch_mode_solve32(g, g->d.spec32, g->d.yh32, s), "correction mode solve launch");
-        if (!rc)
-            rc = dev_rc(launch_fft_inverse32(g, g->d.spec32, phi, it > 0, it == steps - 1, g->d.rstat + 2 * it, s),
-                        "correction inverse FFT launch");
-    }
-    if (!rc) rc = dev_rc(launch_zaxphi(g, npoint, klocat_in, phi, false, s), "ZAXPHI launch");
-    return rc;
+    const Fields F = one_field(g, rho, phi, coef, klocat_in);
+    return run_refined_fields(g, std::vector<chy_grid *>{g}, F, s, maxtrm, isym, redge, npoint, coef_mode, steps,
+                              with_stats);
 }
 
 static int read_diag(chy_grid *g, chy_diag *diag, bool tmass, bool zax)
@@ -530,9 +578,11 @@ int chy_ctx_workers(chy_ctx *ctx, int n)
     return CHY_OK;
 }
 
-int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n)
+// chy_solve_batch (steps = 0) and chy_solve_batch_refined (steps > 0)
+static int solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n, int steps)
 {
     if (!ctx || n < 0 || (n > 0 && !items)) return CHY_EINVAL;
+    if (steps < 0 || steps > kMaxRefine) return CHY_EINVAL;
     if (n == 0) return CHY_OK;
     // validate everything before anything is launched
     bool any_diag = false;
@@ -548,6 +598,8 @@ int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n)
     }
     DeviceScope ds(ctx->device);
     int rc = ensure_workers(ctx);
+    if (rc) return rc;
+    for (int i = 0; i < n && steps > 0 && rc == CHY_OK; ++i) rc = ensure_refine(items[i].grid);
     if (rc) return rc;
     if (any_diag && ctx->diag_cap < n) {
         if (ctx->d_diag) (void)hipFree(ctx->d_diag);
@@ -620,17 +672,23 @@ int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n)
                 }
             }
             const int np = std::min(lead.npoint, 10);
-            if (rc == CHY_OK && grp.size() == 1) {
+            if (rc == CHY_OK && grp.size() == 1 && steps == 0) {
                 rc = run_solve(lead.grid, s, lead.maxtrm, lead.isym, lead.redge, np, lead.rho_d, lead.phi_d,
                                lead.coef_d, lead.coef_mode, lead.diag ? lead.diag->klocat : 0);
             } else if (rc == CHY_OK) {
                 Fields F{};
                 F.nf = (int)grp.size();
+                std::vector<chy_grid *> grids(F.nf);
                 for (int f = 0; f < F.nf; ++f) {
                     const chy_solve_item &it = items[grp[f]];
                     set_field(F, f, it.grid, it.rho_d, it.phi_d, it.coef_d, it.diag ? it.diag->klocat : 0);
+                    grids[f] = it.grid;
                 }
-                rc = run_solve_fields(lead.grid, F, s, lead.maxtrm, lead.isym, lead.redge, np, lead.coef_mode);
+                if (steps > 0)
+                    rc = run_refined_fields(lead.grid, grids, F, s, lead.maxtrm, lead.isym, lead.redge, np,
+                                            lead.coef_mode, steps, false);
+                else
+                    rc = run_solve_fields(lead.grid, F, s, lead.maxtrm, lead.isym, lead.redge, np, lead.coef_mode);
             }
             for (size_t f = 0; f < grp.size() && rc == CHY_OK; ++f) {
                 const chy_solve_item &it = items[grp[f]];
@@ -663,6 +721,14 @@ int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n)
         items[i].diag->klocat = (int)ctx->h_diag[3 * i + 2];
     }
     return CHY_OK;
+}
+
+int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n) { return solve_batch(ctx, items, n, 0); }
+
+int chy_solve_batch_refined(chy_ctx *ctx, const chy_solve_item *items, int n, int steps)
+{
+    if (steps < 1) return CHY_EINVAL;
+    return solve_batch(ctx, items, n, steps);
 }
 
 int chy_ctx_profile(chy_ctx *ctx, int enable)
@@ -744,18 +810,23 @@ int chy_grid_mode_solver(const chy_grid *g) { return g ? g->mode_solver : -1; }
 int chy_grid_set_tune(chy_grid *g, int param, int value)
 {
     if (!g) return CHY_EINVAL;
-    if (param != CHY_TUNE_CR_LANES || !cr_lanes_ok(g, value)) return CHY_EINVAL;
-    if (g->cr_lanes == value) return CHY_OK;
+    int *slot = nullptr;
+    if (param == CHY_TUNE_CR_LANES && cr_lanes_ok(g, value)) slot = &g->cr_lanes;
+    if (param == CHY_TUNE_REFINE_M64 && value >= -1 && value <= g->N) slot = &g->refine_m64;
+    if (!slot) return CHY_EINVAL;
+    if (*slot == value) return CHY_OK;
     for (auto &e : g->graphs) retire_graph(e);
     g->graphs.clear();
-    g->cr_lanes = value;
+    *slot = value;
     return CHY_OK;
 }
 
 int chy_grid_tune(const chy_grid *g, int param)
 {
-    if (!g || param != CHY_TUNE_CR_LANES) return -1;
-    return g->cr_lanes;
+    if (!g) return -1;
+    if (param == CHY_TUNE_CR_LANES) return g->cr_lanes;
+    if (param == CHY_TUNE_REFINE_M64) return g->refine_m64;
+    return -1;
 }
 
 int chy_grid_destroy(chy_grid *g)
@@ -870,7 +941,7 @@ int chy_solve_refined(chy_grid *g, int maxtrm, int isym, double redge, int npoin
     DeviceScope ds(g->ctx->device);
     hipStream_t s = g->ctx->stream;
     rc = run_refined(g, s, maxtrm, isym, redge, npoint, rho_d, phi_d, coef_d, coef_mode, steps,
-                     diag ? diag->klocat : 0);
+                     diag ? diag->klocat : 0, rel_corr != nullptr);
     if (rc == CHY_OK && rel_corr) {
         unsigned long long h[2 * kMaxRefine];
         hipError_t e = hipMemcpyAsync(h, g->d.rstat, sizeof(h), hipMemcpyDeviceToHost, s);
@@ -878,6 +949,7 @@ int chy_solve_refined(chy_grid *g, int maxtrm, int isym, double redge, int npoin
         if (e != hipSuccess) return dev_rc((int)e, "refinement statistics readback");
         rel_corr[0] = 1.0;  // the first step's correction is the whole solution
         for (int it = 1; it < steps; ++it) {
+            // max |D| of the step over max |X| after it (azimuthal spectrum)
             double dm, pm;
             std::memcpy(&dm, &h[2 * it], sizeof(double));
             std::memcpy(&pm, &h[2 * it + 1], sizeof(double));

@@ -96,17 +96,17 @@ struct DeviceGrid {
     double *am = nullptr;        // [JP]
     double *cm = nullptr;        // [JP]
     double *bm = nullptr;        // [(N+1)][JP]
-    // the same operator in physical space for refinement residuals:
-    // C(j) = 1/(RHF DTHETA)^2, AN (AN(1)=0), CN (CN(NZ)=0; CNMAX separate)
-    double *rc_c = nullptr;      // [M]
+    // the z operator for the refinement's spectral residuals: AN (AN(1)=0),
+    // CN (CN(NZ)=0; CNMAX separate), BN
     double *rz_an = nullptr;     // [NZ]
     double *rz_cn = nullptr;     // [NZ]
+    double *rz_bn = nullptr;     // [NZ] BN (BN(1) = -CN(1); the spectral residual)
     // cyclic reduction programs (CrPlan, uploaded)
     CrOp *cr_ops = nullptr;
     CrProg *cr_progs = nullptr;
     CrRow *cr_rows = nullptr;
-    // refinement buffers (allocated on first use)
-    float *res32 = nullptr;      // residual, PHI layout
+    // refinement buffers (allocated on first use): the binary32 right-hand
+    // side / residual spectrum and eigen coordinates, the step statistics
     float *spec32 = nullptr;     // [L][KMAX][JMAX]
     float *yh32 = nullptr;       // [L][PP][JP]
     unsigned long long *rstat = nullptr;  // [2 * kMaxRefine]: max |dPHI|, max |PHI| per step
@@ -223,6 +223,7 @@ struct chy_grid {
     chy::CrPlan cr;              // per-mode cyclic reduction (host copy of the phases)
     int mode_solver = CHY_MODES_CR;  // CHY_MODES_CR or CHY_MODES_EIGEN
     int cr_lanes = 64;               // lanes per r system in the cyclic reduction (64 or 32)
+    int refine_m64 = 4;              // refinement: orders m <= this solved in binary64
     size_t device_bytes = 0;
     std::vector<void *> allocations;
     chy::DeviceGrid d;
@@ -269,6 +270,9 @@ struct Fields {
     double *dscal[kMaxBatch];    // diagnostics
     unsigned *ticket[kMaxBatch]; // ZAXPHI ticket counter
     double *zsamp[kMaxBatch];    // ZAXPHI's samples PHI(J=2..6,K,L), copied by the inverse FFT
+    unsigned long long *rstat[kMaxBatch];  // refinement statistics (max |dPHI|, max |PHI| per step)
+    void *aux[kMaxBatch];        // a third per-field array where a kernel needs one (refinement: the
+                                 // binary64 solution spectrum)
     int klocat_in[kMaxBatch];    // KLOCAT before the call
 };
 // one field on grid g's own workspace
@@ -286,13 +290,7 @@ int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s);
 int launch_fft_forward_f(const chy_grid *g, const Fields &F, int coef_mode, int lanes, bool moments,
                          bool spectrum, hipStream_t s);
 int launch_fft_inverse_f(const chy_grid *g, const Fields &F, hipStream_t s);
-// Refinement (binary32 storage): residual (PHI layout, interior cells) ->
-// spectrum of the interior lanes; spectrum -> PHI interior, set or added to,
-// with POT3's ghost cells when `ghosts`; stat (if not null) receives
-// max |correction| and max |PHI| as the bit patterns of non-negative doubles.
-int launch_fft_forward_res32(const chy_grid *g, const float *res, float *spec32, hipStream_t s);
-int launch_fft_inverse32(const chy_grid *g, const float *spec32, double *phi, bool accumulate, bool ghosts,
-                         unsigned long long *stat, hipStream_t s);
+
 // Mode solves on d.spec; rhs_edges applies the boundary terms of Y first
 // (skipped when the spectral BDYGEN has applied them).
 int launch_mode_solve(const chy_grid *g, bool rhs_edges, hipStream_t s, Profiler *prof);
@@ -303,15 +301,29 @@ bool cr_supported(const chy_grid *g);
 bool cr_lanes_ok(const chy_grid *g, int lanes);
 int launch_cr_solve_f(const chy_grid *g, const Fields &F, hipStream_t s);
 int launch_mode_solve_f(const chy_grid *g, const Fields &F, bool rhs_edges, hipStream_t s, Profiler *prof);
-// The same on binary32 spectrum/eigen-coordinate buffers (refinement
-// corrections: zero boundary values, so no boundary terms).
-int launch_mode_solve32(const chy_grid *g, float *spec32, float *yh32, hipStream_t s);
-// FP64 residual 4 pi G RHO - A PHI of the 7-point operator the mode solves
-// invert, on the interior cells, rounded to binary32 into d.res32 (PHI
-// layout).  first: the interior of PHI is taken as 0 (only the boundary
-// values are read).  clear_stat: also zero d.rstat.
-int launch_residual(const chy_grid *g, const double *rho, const double *phi, bool first, bool clear_stat,
-                    hipStream_t s);
+// The refinement's correction solves (spectral refinement of the mode
+// systems): right-hand side F.spec (binary64 when rhs64, else binary32,
+// [q][k][j] pitch J), eigen coordinates F.yh (binary32); the z transforms in
+// binary32 on the FP32 matrix cores, the r solves in binary32 except for
+// orders m <= m_fp64, which are solved in binary64; the correction X goes to
+// the binary64 solution spectrum F.aux per epi (EPI_XSET: X, EPI_XADD: += X,
+// EPI_XFIN: F.spec = F.aux + X).  refine_supported(g): KMAX-1 <= 127.
+// Ff carries the forward transform's arrays, Fi the inverse's.  stat_step >= 0:
+// the inverse also collects max |X| and max |F.aux + X| into
+// F.rstat + 2 * stat_step.
+enum { EPI_SPEC = 0, EPI_XSET = 1, EPI_XADD = 2, EPI_XFIN = 3 };
+int refine_supported(const chy_grid *g);
+int launch_mode_solve32_f(const chy_grid *g, const Fields &Ff, const Fields &Fi, int m_fp64, bool rhs64, int epi,
+                          int stat_step, hipStream_t s);
+// Zero the first n words of every field's F.rstat (a fill kernel: callers may
+// capture the sequence into a graph).
+int launch_zero_u64_f(const Fields &F, int n, hipStream_t s);
+// The boundary terms of Y on the spectrum (pot3.f:212-224), for every field.
+int launch_rhs_edges_f(const chy_grid *g, const Fields &F, hipStream_t s);
+// The mode systems' binary64 residual B - A X in the spectrum (F.spec = B,
+// F.aux = X) rounded to binary32 into F.yh ([q][k][j], pitch J).
+int launch_spec_residual_f(const chy_grid *g, const Fields &F, hipStream_t s);
+
 // ZAXPHI with PHAC/PHICHK/KLOCAT (POT3's ghost cells come with the inverse FFT).
 // samples: read PHI(J=2..6,K,L) from the copy the inverse FFT just made
 // (only where zaxphi_from_samples(g)), else from PHI itself.

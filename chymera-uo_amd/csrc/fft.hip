@@ -107,6 +107,7 @@ struct FftArgs {
     double ys;            // factor on the interior lanes' spectrum (4 pi G)
     int ghosts;           // inverse: also write POT3's axis/midplane ghost cells
     unsigned long long *stat;  // inverse with accumulation: max |dPHI|, max |PHI| (as bits), or null
+    int stat_step;             // refinement step whose statistics pair stat points at
     double *zsamp;        // inverse: [KMAX][kZaxSamples][L] copy of PHI(J=2..6,K,L) for ZAXPHI, or null
 };
 
@@ -137,6 +138,7 @@ __device__ __forceinline__ void bind(FftArgs &a, const Fields &F, int f)
     if (a.spec) a.spec = F.spec[f];
     if (a.mom) a.mom = F.mom[f];
     if (a.zsamp) a.zsamp = F.zsamp[f];
+    if (a.stat) a.stat = F.rstat[f] + 2 * a.stat_step;
     a.coef = a.coef_mode == CHY_COEF_NONE ? nullptr : F.coef[f];
 }
 
@@ -450,9 +452,9 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a, const Fields 
     }
 }
 
-// transform variants: the FP64 solve, and the refinement's binary32-storage
-// correction (residual in, spectrum in binary32, PHI updated in place)
-enum { FWD64 = 0, FWD32 = 1, INV64 = 2, INV32_SET = 3, INV32_ADD = 4 };
+// transform variants (the refinement works in the spectrum and uses the
+// binary64 transforms)
+enum { FWD64 = 0, INV64 = 2 };
 
 template <int N1, int N2, int NL, int KIND>
 int launch_one(const FftArgs &a, const Fields &F, hipStream_t s)
@@ -461,10 +463,7 @@ int launch_one(const FftArgs &a, const Fields &F, hipStream_t s)
     const int blocks = (chunks + kXcds - 1) / kXcds * kXcds;
     const dim3 grid(blocks, F.nf), block(NL * N2);
     if (KIND == FWD64) hipLaunchKernelGGL((k_fft_forward<N1, N2, NL, double, double>), grid, block, 0, s, a, F);
-    if (KIND == FWD32) hipLaunchKernelGGL((k_fft_forward<N1, N2, NL, float, float>), grid, block, 0, s, a, F);
     if (KIND == INV64) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, double, false>), grid, block, 0, s, a, F);
-    if (KIND == INV32_SET) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, float, false>), grid, block, 0, s, a, F);
-    if (KIND == INV32_ADD) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, float, true>), grid, block, 0, s, a, F);
     return (int)hipGetLastError();
 }
 
@@ -540,30 +539,6 @@ int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s)
 {
     const Fields F = one_field(g, nullptr, phi, nullptr, 0);
     return launch_fft_inverse_f(g, F, s);
-}
-
-int launch_fft_forward_res32(const chy_grid *g, const float *res, float *spec32, hipStream_t s)
-{
-    Fields F = one_field(g, res, nullptr, nullptr, 0);
-    F.spec[0] = spec32;
-    FftArgs a = make_args(g, LANES_INTERIOR);
-    a.rho = res;
-    a.spec = spec32;
-    a.ys = 1.0;  // the residual already carries 4 pi G
-    return dispatch<FWD32>(g->N, a, F, s);
-}
-
-int launch_fft_inverse32(const chy_grid *g, const float *spec32, double *phi, bool accumulate, bool ghosts,
-                         unsigned long long *stat, hipStream_t s)
-{
-    Fields F = one_field(g, nullptr, phi, nullptr, 0);
-    F.spec[0] = const_cast<float *>(spec32);
-    FftArgs a = make_args(g, LANES_INTERIOR);
-    a.spec = const_cast<float *>(spec32);
-    a.phi_out = phi;
-    a.ghosts = ghosts ? 1 : 0;
-    a.stat = stat;
-    return accumulate ? dispatch<INV32_ADD>(g->N, a, F, s) : dispatch<INV32_SET>(g->N, a, F, s);
 }
 
 }  // namespace chy

@@ -456,10 +456,10 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
     CHY_TRY(upload(g, &d.cr_ops, g->cr.ops));
     CHY_TRY(upload(g, &d.cr_progs, g->cr.progs));
     CHY_TRY(upload(g, &d.cr_rows, g->cr.rows));
-    // the 7-point operator of the same system for refinement residuals
-    CHY_TRY(upload(g, &d.rc_c, c));
+    // the z operator of the same system for the refinement's spectral residuals
     CHY_TRY(upload(g, &d.rz_an, an));
     CHY_TRY(upload(g, &d.rz_cn, cn));
+    CHY_TRY(upload(g, &d.rz_bn, bn));
     CHY_TRY(upload(g, &d.ring_of_pos, order));
     CHY_TRY(upload(g, &d.tin, tin));
     CHY_TRY(upload(g, &d.tout, tout));

@@ -121,14 +121,17 @@ enum { CHY_MODES_CR = 0, CHY_MODES_EIGEN = 1 };
 int chy_grid_set_mode_solver(chy_grid *g, int kind);
 int chy_grid_mode_solver(const chy_grid *g);
 
-/* Kernel shapes of a grid (no reference counterpart: launch configuration).
- *   CHY_TUNE_CR_LANES  lanes per r system in the cyclic reduction: 64 (one
- *                      order per wavefront) or 32 (two orders per wavefront,
- *                      JMAX-1 <= 256)
- * Results do not depend on the shapes beyond rounding.  Changing a shape
+/* Kernel shapes and precision split of a grid (no reference counterpart).
+ *   CHY_TUNE_CR_LANES    lanes per r system in the cyclic reduction: 64 (one
+ *                        order per wavefront) or 32 (two orders per
+ *                        wavefront, JMAX-1 <= 256)
+ *   CHY_TUNE_REFINE_M64  refined solves: the r solves of orders m <= this
+ *                        value run in binary64, the others in binary32
+ *                        (-1..LMAX/2; default 4)
+ * Results do not depend on the shapes beyond rounding.  Changing a value
  * drops the grid's captured solve graphs.  CHY_EINVAL for a value the grid
  * cannot take. */
-enum { CHY_TUNE_CR_LANES = 0 };
+enum { CHY_TUNE_CR_LANES = 0, CHY_TUNE_REFINE_M64 = 1 };
 int chy_grid_set_tune(chy_grid *g, int param, int value);
 int chy_grid_tune(const chy_grid *g, int param);
 
@@ -201,19 +204,26 @@ int chy_pot3_host(chy_grid *g, int npoint, const double *rho_h, double *phi_h, d
                   int coef_mode, chy_diag *diag);
 
 /* ---- refined solve (no reference counterpart) ----
- * The same results as chy_solve (PHI, COEF, diagnostics) by iterative
- * refinement: BDYGEN and COEF in binary64 as chy_solve computes them, then
- * `steps` (1..8) correction steps -- the binary64 residual of the discrete
- * 7-point operator the mode solves invert, a correction solved with binary32
- * spectrum and eigen-coordinate storage (binary64 arithmetic in the kernels,
- * half the memory traffic), added to PHI in binary64 -- then POT3's ghost
- * cells and ZAXPHI.  The first step starts from 0 in the interior (whatever
- * phi_d held there is not read).  rel_corr (host, may be NULL) receives, per
- * step, the largest correction relative to the largest |PHI| after it (1 for
- * the first step).  Waits when diag or rel_corr is non-NULL. */
+ * The same results as chy_solve (PHI, COEF, diagnostics) to a stated
+ * accuracy, by iterative refinement: BDYGEN and COEF in binary64 as
+ * chy_solve computes them, then `steps` (1..8) correction steps -- the
+ * binary64 residual of the discrete 7-point operator the mode solves invert,
+ * rounded to binary32; its azimuthal transform (binary32 in and out); the
+ * mode solves in binary32 (z transforms on the FP32 matrix cores, r solves
+ * in binary32 except the orders m <= CHY_TUNE_REFINE_M64, which are solved in
+ * binary64); the inverse transform added to PHI in binary64 -- then POT3's
+ * ghost cells and ZAXPHI.  The first step starts from 0 in the interior
+ * (whatever phi_d held there is not read).  rel_corr (host, may be NULL)
+ * receives, per step, the largest correction relative to the largest |PHI|
+ * after it (1 for the first step).  Waits when diag or rel_corr is non-NULL. */
 int chy_solve_refined(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho_d,
                       double *phi_d, double *coef_d, int coef_mode, int steps, chy_diag *diag,
                       double *rel_corr);
+/* chy_solve_batch with every item solved as chy_solve_refined with `steps`
+ * correction steps (items grouped and launched together as in
+ * chy_solve_batch; each item's results are bitwise those of its own
+ * chy_solve_refined). */
+int chy_solve_batch_refined(chy_ctx *ctx, const chy_solve_item *items, int n, int steps);
 
 /* ---- ZAXPHI alone (pot/pot3.f:311-399) ----
  * The NPOINT-point extrapolation of PHI to the z axis from PHI(J=2..,K,L) and

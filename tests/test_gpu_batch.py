@@ -195,3 +195,47 @@ def test_batched_launches_match_oracle(ctx):
     finally:
         for jb in jobs:
             jb["grid"].close()
+
+
+def test_config4_64_heterogeneous_fields_refined(ctx):
+    """The scope's batched configuration: 64 independent fields drawn from a
+    seed (JMAX in {64,128,256}, KMAX in {32,64,128}, LMAX in {64,128,256},
+    tori of random centre, radius and index with log-normal noise), each on
+    its own grid object, solved in one chy_solve_batch_refined call (fields
+    on identical grids share launches).  Every field against its own plain
+    FP64 solve (1e-10), a sample bitwise against its own refined solve, and
+    the smallest fields against the oracle's solve."""
+    import pyoracle as po
+    jobs = []
+    for i in range(64):
+        g, rho_d = fields.random_field_device(1000 + i)
+        pg = chypot.PreparedGrid(ctx, g)
+        jobs.append(dict(grid=pg, g=g, rho=rho_d, phi=torch.zeros_like(rho_d),
+                         coef=torch.zeros(pg.coef_shape, dtype=torch.float64, device="cuda"),
+                         coef_mode=chypot.COEF_K2))
+    shapes = {(j["g"].jmax, j["g"].kmax, j["g"].lmax) for j in jobs}
+    print(f"[gpu] config 4: 64 fields on {len(shapes)} distinct grids", flush=True)
+    ctx.solve_batch(jobs, refine_steps=2)
+    ctx.sync()
+    worst = 0.0
+    for i, j in enumerate(jobs):
+        plain = torch.zeros_like(j["rho"])
+        j["grid"].solve(j["rho"], plain)
+        ctx.sync()
+        e = float((j["phi"] - plain).abs().max() / plain.abs().max())
+        worst = max(worst, e)
+        if i % 8 == 0:
+            alone = torch.zeros_like(j["rho"])
+            j["grid"].solve_refined(j["rho"], alone, steps=2)
+            ctx.sync()
+            assert torch.equal(alone, j["phi"]), i
+    print(f"[gpu] config 4: worst refined vs plain FP64 {worst:.2e}", flush=True)
+    assert worst < 1e-10
+    small = sorted(range(64), key=lambda i: jobs[i]["g"].jmax * jobs[i]["g"].kmax * jobs[i]["g"].lmax)[:3]
+    for i in small:
+        g = jobs[i]["g"]
+        rho = chypot.to_host(jobs[i]["rho"])
+        want = po.solve(g, rho)
+        assert rel_max(chypot.to_host(jobs[i]["phi"]), want["phi"]) < 1e-10, g.name
+    for j in jobs:
+        j["grid"].close()

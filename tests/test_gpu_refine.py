@@ -1,6 +1,9 @@
-"""Refined solve (chy_solve_refined): binary32 correction solves on a binary64
-residual must reproduce the plain binary64 solve to 1e-10 (max relative
-PHI error), with the reference's boundary values, COEF and diagnostics.
+"""Refined solve (chy_solve_refined, chy_solve_batch_refined): iterative
+refinement of the mode systems in the azimuthal spectrum -- binary64
+residuals, binary32 correction solves on the FP32 matrix cores with binary64
+r solves for the orders m <= 4 -- must reproduce the plain binary64 solve to
+1e-10 (max relative PHI error), with the reference's boundary values, COEF
+and diagnostics; the error is reported per refinement step.
 """
 import numpy as np
 import pytest
@@ -74,6 +77,74 @@ def test_refined_256_against_plain_and_reference(ctx, steps):
           f"vs reference {e_ref:.2e}", flush=True)
     assert e_plain < 1e-10
     assert e_ref < 1e-10
+
+
+def _plain(ctx, pg, rho_d):
+    out = torch.zeros_like(rho_d)
+    pg.solve(rho_d, out)
+    ctx.sync()
+    return out
+
+
+@pytest.mark.parametrize("density", ["config1_torus", "spiral"])
+def test_refined_error_per_step_256(ctx, density):
+    """Config 3: the config-1 torus (and the spiral of config 2) at
+    256x128x256.  The error against the plain binary64 solve after each
+    refinement step: the first step (binary32 solves) is ~1e-6, the second
+    below 1e-10; then it settles at the distance between the exact solution
+    of the mode systems and the plain path's cyclic reduction (its
+    reference-faithful root tables), ~2e-12."""
+    g = fields.unit_grid(256, 128, 256)
+    rho = fields.density_small_torus(g, 1) if density == "config1_torus" else fields.density_spiral_torus(g, 2)
+    pg = chypot.PreparedGrid(ctx, g)
+    assert pg.refine_m64 == 4
+    rho_d = chypot.to_device(rho)
+    plain = _plain(ctx, pg, rho_d)
+    scale = float(plain.abs().max())
+    errs = []
+    for steps in range(1, 5):
+        phi = torch.zeros_like(rho_d)
+        pg.solve_refined(rho_d, phi, steps=steps)
+        ctx.sync()
+        errs.append(float((phi - plain).abs().max()) / scale)
+    pg.close()
+    print(f"[gpu] refined {density} 256x128x256: error vs plain FP64 per step " +
+          " ".join(f"{e:.2e}" for e in errs), flush=True)
+    assert errs[0] < 1e-5
+    assert errs[1] < 1e-10
+    assert max(errs[2:]) < 1e-11
+
+
+def test_batched_refined_equals_single(ctx):
+    """chy_solve_batch_refined over fields on identical grids (one launch per
+    kernel) and on another grid: each field bitwise its own refined solve."""
+    gs = [fields.unit_grid(64, 32, 64)] * 3 + [fields.unit_grid(32, 16, 64)]
+    pgs = [chypot.PreparedGrid(ctx, g) for g in gs]
+    rhos = [chypot.to_device(fields.density_spiral_torus(g, seed=10 + i)) for i, g in enumerate(gs)]
+    jobs = [dict(grid=pg, rho=r, phi=torch.zeros_like(r),
+                 coef=torch.zeros(pg.coef_shape, dtype=torch.float64, device="cuda"), coef_mode=chypot.COEF_K2)
+            for pg, r in zip(pgs, rhos)]
+    diags = ctx.solve_batch(jobs, diag=True, refine_steps=2)
+    for i, (pg, j) in enumerate(zip(pgs, jobs)):
+        phi = torch.zeros_like(j["rho"])
+        coef = torch.zeros_like(j["coef"])
+        d, _ = pg.solve_refined(j["rho"], phi, steps=2, coef=coef, coef_mode=chypot.COEF_K2, diag=True)
+        ctx.sync()
+        assert torch.equal(phi, j["phi"]), i
+        assert torch.equal(coef, j["coef"]), i
+        assert d.tmass == diags[i].tmass and d.klocat == diags[i].klocat
+    for pg in pgs:
+        pg.close()
+
+
+def test_refined_needs_kmax_at_most_128(ctx):
+    g = fields.unit_grid(16, 256, 16)
+    pg = chypot.PreparedGrid(ctx, g)
+    rho = torch.zeros(pg.field_shape, dtype=torch.float64, device="cuda")
+    with pytest.raises(chypot.ChyError) as ei:
+        pg.solve_refined(rho, torch.zeros_like(rho), steps=2)
+    assert ei.value.code == chypot.CHY_EUNSUPPORTED
+    pg.close()
 
 
 def test_refined_rejects_bad_steps(ctx):
