@@ -1,39 +1,45 @@
 #!/usr/bin/env python3
 """Potential-solve benchmark on one MI355X.
 
+Metric: pot3 solves/sec at 256x128x256, the refined solve against the plain
+FP64 solve, at 1e-10 max relative PHI error; p50/p99 latency.
+
 Workload: the 256x128x256 spiral-arm torus (JMAX=256, KMAX=128, LMAX=256,
-DELR=DELZ=1/256, seeded), plain FP64.  One solve is BDYGEN + POT3 on one
-density field, device resident, COEF for the K=2 plane.  A step is one batch
-of B independent fields (different seeds, each on its own prepared grid)
-solved by one chy_solve_batch call, which runs them as batched launches of
-up to 32 fields (default B=32: one launch per kernel); `value` is solves per
-second over the timed steps (B solves per step) with every density already in
-HBM.  B=1 is the single-field solve.
+DELR=DELZ=1/256, seeded), one field per prepared grid, every density already
+resident in HBM, COEF for the K=2 plane.  One solve is BDYGEN + POT3 on one
+field.  A step is one batched call over B fields (default 32, one launch per
+kernel for all of them):
 
-The line also carries:
-  * p50/p99 latency of one solve (a separate pass of single-field chy_solve
-    calls timed with HIP events on the solve's stream; --latency-steps 0 skips
-    it) and of one batch step;
-  * the error of field 0 against the fixture solutions (reference FP64 and the
-    80-bit solve of the same discrete system);
-  * the roofline of the dominant kernel of the timed region: HIP events
-    around every stage on the worker stream the batched launches run on give
-    its average launch duration (`batch_stages_ms_per_launch`), one launch
-    processes `fields_per_launch` fields; `roofline.traffic` is the HBM bytes
-    per launch that rocprofv3 FETCH_SIZE/WRITE_SIZE recorded for that kernel
-    with launches of that many fields (profiles/potsolve_256x128x256_batch<B>_
-    hbm_traffic.txt, or ..._hbm_traffic.txt for single fields), or null;
-  * `stages_ms`: per-stage times of single-field solves launched kernel by
-    kernel (--stage-steps 0 skips them);
-  * the reference CPU solve timed on this host;
-  * `source_grav`: the hydro step's momentum sources that read PHI
-    (radhydro/source.f:41-79) on the same grid, per call, with the oracle's
-    C loop timed beside it, and the star potential added to PHI before them
-    (pot/ExternalPot.f, `source_grav.external_pot`) and the disk's pull on
-    the star (find_force of pot/wiggle.F, `source_grav.find_force`)
-    (--source-calls 0 skips them).
+  * `value`: solves/s of the refined solve (chy_solve_batch_refined: iterative
+    refinement of the mode systems in the azimuthal spectrum, binary32 mode
+    solves on the FP32 matrix cores, binary64 residuals and binary64 r solves
+    for the orders m <= 4, --refine-steps correction steps), timed over
+    --steps steps after --warmup;
+  * `plain_fp64`: the same for the plain FP64 solve (chy_solve_batch, BLKTRI's
+    cyclic reduction with the reference's root tables -- the path that
+    matches the reference to 1e-12), and `plain_fp64_eigen` for the opt-in z
+    eigenbasis solver; `vs_plain` = value / plain_fp64 solves/s;
+  * errors: the refined result against the plain FP64 one (the 1e-10
+    criterion) and against the reference's fixture planes, and per
+    refinement step (1..4) against the plain FP64 result;
+  * p50/p99 latency of single-field solves (HIP events on the solve stream)
+    for both paths;
+  * `rooflines`: every stage of both paths in their timed regions (HIP events
+    around each stage on the stream the batched launches run on): algorithmic
+    bytes per field (the scope table's figures: forward transform 134.2 MB,
+    cyclic reduction 133.4 MB, inverse 135.8 MB at 256x128x256) over the
+    stage time, against 8 TB/s; the z transforms also against the matrix-core
+    peak; `roofline` is the dominant stage of the headline (refined) timed
+    region, with `traffic` from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    of that kernel when profiles/ holds one;
+  * `hetero_batch` (64 fields of mixed grids, refined) and `serving` (Poisson
+    arrivals over 60% 128x64x128, 30% 256x128x256, 10% 512x256x512 under an
+    HBM budget) -- both on by default (--hetero 0 / --serve 0 skip them);
+  * `cpu_baseline`: the reference's own BDYGEN+POT3 (pot/*.f built with flang
+    -O3 -fdefault-real-8 under oracle/_ref) on this host's cores and on one
+    core, 3 warm-ups and --cpu-reps timed solves each, p50/p99.
 
-    python bench.py [--steps K] [--warmup W] [--batch B] [--workers W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
 """
 from __future__ import annotations
 
@@ -53,42 +59,52 @@ sys.path.insert(0, str(ROOT / "chymera-uo_amd"))
 
 METRIC = "pot3 solves/sec @256x128x256 refined vs plain FP64 at 1e-10; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
-FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 (matrix and vector)
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 (matrix)
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 (matrix)
 
 
-def stage_work(g):
-    """Algorithmic bytes / flops of one launch of each stage at grid g."""
+def stage_work(g, path, steps):
+    """Algorithmic bytes (and matrix-core flops) per field of one solve, per
+    stage, summed over the stage's launches in that solve.  Transforms and
+    the cyclic reduction follow the scope table: forward 2 J K L 8 bytes,
+    reduction J K L 8 + M NZ L 8, inverse J K L 8 + (J+2)(K+2) L 8."""
     J, K, L = g.jmax, g.kmax, g.lmax
     M, NZ = J - 1, K - 1
-    lanes = J * K * L * 8              # every (J,K) lane x L samples, FP64
-    modes = M * NZ * L * 8             # interior mode unknowns
-    gemm = 2.0 * NZ * NZ * M * L       # one z transform
-    return {
-        "bdygen": dict(bytes=modes, flops=None),
-        "fft_fwd": dict(bytes=2 * lanes, flops=None),
-        "zfwd": dict(bytes=lanes + modes, flops=gemm),
-        "rsolve": dict(bytes=2 * modes, flops=None),
-        "zinv": dict(bytes=2 * modes, flops=gemm),
-        "fft_inv": dict(bytes=2 * lanes, flops=None),
-        "ghost": dict(bytes=None, flops=None),
-        "fft_bdy": dict(bytes=None, flops=None),
-    }
+    PP = ((NZ + 63) // 64) * 64
+    JP = 64 * (1 << max(0, int(np.ceil(np.log2(max(1, M / 64))))))
+    lanes = J * K * L * 8
+    modes = M * NZ * L * 8
+    gemm = 2.0 * NZ * NZ * M * L
+    w = {"fft_fwd": dict(bytes=2 * lanes, flops=None),
+         "fft_inv": dict(bytes=lanes + (J + 2) * (K + 2) * L * 8, flops=None),
+         "bdygen": dict(bytes=None, flops=None), "ghost": dict(bytes=None, flops=None)}
+    if path == "cr":
+        w["cr"] = dict(bytes=lanes + modes, flops=None)
+    elif path == "eigen":
+        w["zfwd"] = dict(bytes=modes + PP * JP * L * 8, flops=gemm)
+        w["rsolve"] = dict(bytes=2 * modes, flops=None)
+        w["zinv"] = dict(bytes=PP * JP * L * 8 + modes, flops=gemm)
+    else:  # refined, binary32 mode solves; the first step reads the binary64 right-hand side
+        m32 = modes // 2
+        w["zfwd"] = dict(bytes=modes + m32 + (steps - 1) * 2 * m32, flops=steps * gemm)
+        w["rsolve"] = dict(bytes=steps * 2 * m32, flops=None)
+        xb = [m32 + modes] + [m32 + 2 * modes] * (steps - 1)  # X written, then read and written
+        w["zinv"] = dict(bytes=sum(xb), flops=steps * gemm)
+        w["resid"] = dict(bytes=(steps - 1) * (2 * modes + m32), flops=None)
+    return w
 
 
 # stage -> kernel whose rocprofv3 counters stand for it
-STAGE_KERNEL = {"zfwd": "k_zgemm_reg<true, 128, 64, 1, double>", "zinv": "k_zgemm_reg<false, 128, 64, 1, double>",
-                "fft_fwd": "k_fft_forward<16, 8, 32, double, double>",
-                "fft_inv": "k_fft_inverse<16, 8, 32, double, false>", "rsolve": "k_rsolve<4, double>"}
+STAGE_KERNEL = {"fft_fwd": "k_fft_forward<16, 8, 32, double, double>",
+                "fft_inv": "k_fft_inverse<16, 8, 32, double, false>"}
 
 
-def hbm_traffic(stage, batch):
+def hbm_traffic(stage, batch, workload):
     """HBM bytes per launch of the stage's kernel as rocprofv3 FETCH_SIZE +
     WRITE_SIZE recorded them, each scaled by what the counter read for a
-    device-to-device copy of known size: profiles/..._hbm_traffic.txt for
-    single-field launches, profiles/..._batch<B>_hbm_traffic.txt for launches
-    of B fields."""
-    name_b = "" if batch == 1 else f"_batch{batch}"
-    f = ROOT / "profiles" / f"potsolve_256x128x256{name_b}_hbm_traffic.txt"
+    device-to-device copy of known size (profiles/<workload>_batch<B>_
+    hbm_traffic.txt), or None."""
+    f = ROOT / "profiles" / f"{workload}_batch{batch}_hbm_traffic.txt"
     name = STAGE_KERNEL.get(stage)
     if name is None or not f.exists():
         return None
@@ -100,42 +116,63 @@ def hbm_traffic(stage, batch):
     return None
 
 
-def hetero_batch(ctx, n, reps=3):
-    """One chy_solve_batch over n independent fields of mixed grids (the grid
-    and torus of fields.random_field for seeds 1000.., generated on the GPU),
-    one prepared grid per shape; fields per second over `reps` batches."""
+def rooflines(stage_times, work, nsolve, fields_per_launch, fp_peak):
+    """Per stage of a timed region of nsolve solves: total ms, launches, and
+    the algorithmic rate against the HBM (and matrix-core) peaks."""
+    out = {}
+    for st, (ms, n) in stage_times.items():
+        if not n or st not in work:
+            continue
+        w = work[st]
+        e = {"ms_total": round(ms, 4), "launches": n, "ms_per_launch": round(ms / n, 5),
+             "fields_per_launch": fields_per_launch}
+        if w["bytes"]:
+            gbs = w["bytes"] * nsolve / (ms * 1e-3) / 1e9
+            e.update(bytes_per_field=w["bytes"], achieved_GBs=round(gbs, 1), frac_hbm=round(gbs / HBM_PEAK_GBS, 4))
+        if w["flops"]:
+            tf = w["flops"] * nsolve / (ms * 1e-3) / 1e12
+            e.update(flops_per_field=w["flops"], achieved_TFLOPs=round(tf, 2), frac_mfma=round(tf / fp_peak, 4))
+        out[st] = e
+    return out
+
+
+def hetero_batch(ctx, n, refine_steps, reps=3):
+    """Config 4: one batched call over n independent fields of mixed grids
+    (fields.random_field_device for seeds 1000.., generated on the GPU), each
+    on its own prepared grid (fields on identical grids share launches);
+    fields per second over `reps` calls, refined and plain."""
     import torch
 
     import chypot
     from chypot import fields
-    prepared, jobs, points = {}, [], 0
+    jobs, points, shapes = [], 0, set()
     for i in range(n):
         g, rho_d = fields.random_field_device(1000 + i)
-        key = (g.jmax, g.kmax, g.lmax)
-        if key not in prepared:
-            prepared[key] = chypot.PreparedGrid(ctx, g)
-        jobs.append(dict(grid=prepared[key], rho=rho_d, phi=torch.zeros_like(rho_d),
-                         coef=torch.zeros((2, g.lmax // 2, g.kmax + 2, g.jmax + 2), dtype=torch.float64,
-                                          device="cuda"),
+        pg = chypot.PreparedGrid(ctx, g)
+        shapes.add((g.jmax, g.kmax, g.lmax))
+        jobs.append(dict(grid=pg, rho=rho_d, phi=torch.zeros_like(rho_d),
+                         coef=torch.zeros(pg.coef_shape, dtype=torch.float64, device="cuda"),
                          coef_mode=chypot.COEF_K2))
         points += g.jmax * g.kmax * g.lmax
-    ctx.solve_batch(jobs)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        ctx.solve_batch(jobs)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / reps
-    out = {"fields": n, "distinct_grids": len(prepared), "batch_ms": dt * 1e3,
-           "fields_per_s": n / dt, "grid_points_per_s": points / dt}
-    for pg in prepared.values():
-        pg.close()
+    out = {"fields": n, "distinct_grids": len(shapes), "refine_steps": refine_steps}
+    for label, steps in (("refined", refine_steps), ("plain_fp64", 0)):
+        ctx.solve_batch(jobs, refine_steps=steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.solve_batch(jobs, refine_steps=steps)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        out[label] = {"batch_ms": dt * 1e3, "fields_per_s": n / dt, "grid_points_per_s": points / dt}
+    for j in jobs:
+        j["grid"].close()
     return out
 
 
 def serve_run(ctx, args):
-    """The serving loop (chypot.serve) over Poisson arrivals: throughput and
-    per-request latency from arrival to completion."""
+    """Config 5: the serving loop (chypot.serve) over Poisson arrivals,
+    refined where the grid allows it: throughput and per-request latency
+    from arrival to completion."""
     import torch
 
     from chypot import serve
@@ -143,16 +180,17 @@ def serve_run(ctx, args):
     dens = {shape: serve.density_pool(shape, 2, seed=17, on_device=True) for shape, _ in mix}
     torch.cuda.synchronize()
     reqs = serve.poisson_requests(args.serve_rate, args.serve, mix, dens, seed=5)
-    srv = serve.Server(ctx, hbm_budget=int(args.serve_budget_gib * (1 << 30)), max_batch=16, workers=8)
-    srv.prepare([shape for shape, _ in mix])
+    srv = serve.Server(ctx, hbm_budget=int(args.serve_budget_gib * (1 << 30)), max_batch=16, workers=8,
+                       refine_steps=args.refine_steps)
+    srv.prepare([shape for shape, _ in mix], per_shape=4)
     t0 = time.perf_counter()
     done = srv.run(reqs)
     wall = time.perf_counter() - t0
     srv.close()
     out = serve.latency_summary(done)
-    out.update({"offered_rate_per_s": args.serve_rate, "seconds": args.serve,
-                "completed_per_s": len(done) / wall,
-                "mix": "60% 128x64x128, 30% 256x128x256, 10% 512x256x512"})
+    out.update({"offered_rate_per_s": args.serve_rate, "seconds": args.serve, "budget_gib": args.serve_budget_gib,
+                "completed_per_s": len(done) / wall, "refine_steps": args.refine_steps,
+                "mix": "60% 128x64x128, 30% 256x128x256, 10% 512x256x512 (refined where KMAX <= 128)"})
     return out
 
 
@@ -231,67 +269,86 @@ def source_row(ctx, stream, g, n, with_cpu):
     return out
 
 
+def _refpot_times(exe, g, rho, threads, nwarm, nrep):
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = pathlib.Path(td) / "in.bin", pathlib.Path(td) / "out.bin"
+        with open(fin, "wb") as f:
+            f.write(np.array([g.jmax, g.kmax, g.lmax, 2, 10, 8, nwarm, nrep], dtype="<i4").tobytes())
+            f.write(np.array([g.delr, g.delz, 0.0, 0.0], dtype="<f8").tobytes())
+            f.write(np.asarray(rho, dtype="<f8").tobytes(order="F"))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="512M")
+        subprocess.run([str(exe), "solve", str(fin), str(fout)], check=True, env=env, stdout=subprocess.DEVNULL)
+        raw = fout.read_bytes()
+    n = int(np.prod(g.shape))
+    nc = (g.jmax + 2) * (g.kmax + 2) * (g.lmax // 2) * 2
+    off = (2 * n + 1 + nc + 3) * 8
+    used = int(np.frombuffer(raw, dtype="<i4", count=1, offset=off)[0])
+    return used, np.frombuffer(raw, dtype="<f8", count=nrep, offset=off + 4).copy()
+
+
+def host_cores():
+    """Cores this process may use (the affinity mask, capped by OMP_NUM_THREADS when set)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(env)) if env and env.isdigit() and int(env) > 0 else n
+
+
 def cpu_baseline(g, rho, nrep):
-    """Reference BDYGEN+POT3 (flang -O3 real-8 build of pot/*.f, oracle/_ref) on
-    this host, 1 thread; falls back to the oracle's C restatement."""
+    """The reference's BDYGEN+POT3 (pot/*.f, flang -O3 -fdefault-real-8,
+    oracle/_ref) on this host: all cores and one core, 3 warm-ups and nrep
+    timed solves each.  Without that build (it is made in a tree holding the
+    reference) it says so on stderr and times the oracle's C restatement."""
     exe = ROOT / "oracle" / "_ref" / f"r8_{g.name}" / "refpot"
+    cores = host_cores()
     if exe.exists():
-        with tempfile.TemporaryDirectory() as td:
-            fin, fout = pathlib.Path(td) / "in.bin", pathlib.Path(td) / "out.bin"
-            with open(fin, "wb") as f:
-                f.write(np.array([g.jmax, g.kmax, g.lmax, 2, 10, 8, 1, nrep], dtype="<i4").tobytes())
-                f.write(np.array([g.delr, g.delz, 0.0, 0.0], dtype="<f8").tobytes())
-                f.write(np.asarray(rho, dtype="<f8").tobytes(order="F"))
-            env = dict(os.environ, OMP_NUM_THREADS="1", OMP_STACKSIZE="512M")
-            subprocess.run([str(exe), "solve", str(fin), str(fout)], check=True, env=env,
-                           stdout=subprocess.DEVNULL)
-            raw = fout.read_bytes()
-        n = int(np.prod(g.shape))
-        nc = (g.jmax + 2) * (g.kmax + 2) * (g.lmax // 2) * 2
-        off = (2 * n + 1 + nc + 3) * 8
-        secs = np.frombuffer(raw, dtype="<f8", count=nrep, offset=off + 4)
-        p50 = float(np.median(secs))
-        return {"value": 1.0 / p50, "unit": "solves/s", "cores": 1, "kind": "reference",
-                "sample": f"{nrep} solves (after 1 warm-up) of the same 256x128x256 field, "
-                          f"BDYGEN+POT3 of pot/*.f built with flang -O3 -fdefault-real-8, 1 thread; "
-                          f"p50 {p50:.3f} s"}
+        res = {}
+        for label, th in (("all_cores", cores), ("one_core", 1)):
+            used, secs = _refpot_times(exe, g, rho, th, 3, nrep)
+            res[label] = {"threads": used, "p50_s": float(np.percentile(secs, 50)),
+                          "p99_s": float(np.percentile(secs, 99)), "solves_per_s": float(1.0 / np.mean(secs))}
+        a = res["all_cores"]
+        return {"value": a["solves_per_s"], "unit": "solves/s", "cores": a["threads"], "kind": "reference",
+                "sample": f"3 warm-ups then {nrep} timed BDYGEN+POT3 solves of the 256x128x256 field by the "
+                          f"reference's pot/*.f (flang -O3 -fdefault-real-8, OpenMP), {a['threads']} threads; "
+                          f"p50 {a['p50_s'] * 1e3:.1f} ms, p99 {a['p99_s'] * 1e3:.1f} ms",
+                "one_core": res["one_core"], "all_cores": a}
+    print(f"bench.py: the reference build {exe} is missing (build it with __graft_entry__.build() where the "
+          "reference tree is present); cpu_baseline times the oracle's C restatement instead", file=sys.stderr)
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    pyoracle.use_cyclic_reduction(True)
     pyoracle.solve(g, rho)
     secs = []
     for _ in range(nrep):
         t = time.perf_counter()
         pyoracle.solve(g, rho)
         secs.append(time.perf_counter() - t)
-    p50 = float(np.median(secs))
-    return {"value": 1.0 / p50, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"{nrep} solves of the same field by the C restatement, {threads} threads"}
+    pyoracle.use_cyclic_reduction(False)
+    return {"value": 1.0 / float(np.mean(secs)), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"{nrep} solves of the same field by the oracle's C restatement (BLKTRI's reduction), "
+                      f"{cores} threads; the reference build was missing"}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=32, help="independent fields per step")
     ap.add_argument("--workers", type=int, default=0, help="worker streams (default min(batch, 8))")
-    ap.add_argument("--refine-steps", type=int, default=2,
-                    help="correction steps of the refined path (0: do not measure it)")
-    ap.add_argument("--hetero", type=int, default=0,
-                    help="also time one batch of this many heterogeneous fields (J in 64..256, "
-                         "K in 32..128, L in 64..256; GPU-generated tori)")
-    ap.add_argument("--serve", type=float, default=0.0,
-                    help="also run the serving loop for this many seconds of Poisson arrivals "
-                         "(60%% 128x64x128, 30%% 256x128x256, 10%% 512x256x512)")
-    ap.add_argument("--serve-rate", type=float, default=2000.0, help="arrivals per second")
+    ap.add_argument("--refine-steps", type=int, default=2, help="correction steps of the refined solve")
+    ap.add_argument("--paths", default="refined,cr,eigen",
+                    help="which paths to time (refined is the headline; a profile may time one)")
+    ap.add_argument("--cpu-reps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--hetero", type=int, default=64,
+                    help="also time one batch of this many heterogeneous fields, refined and plain (0: skip)")
+    ap.add_argument("--serve", type=float, default=0.5,
+                    help="also run the serving loop for this many seconds of Poisson arrivals (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="arrivals per second")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="HBM budget of the server")
     ap.add_argument("--latency-steps", type=int, default=-1,
-                    help="single-field chy_solve calls for p50/p99 (default: --steps; 0 skips)")
-    ap.add_argument("--stage-steps", type=int, default=-1,
-                    help="single-field stage-timed solves for stages_ms (default: --steps; 0 skips)")
+                    help="single-field solves per path for p50/p99 (default: 2 x --steps; 0 skips)")
     ap.add_argument("--source-calls", type=int, default=20,
                     help="also time this many calls of the SOURCE momentum sources (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -302,6 +359,7 @@ def main():
         print("bench.py measures one GPU: the potential solve of one field runs on one MI355X; "
               "splitting a field over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
+    paths = [p for p in args.paths.split(",") if p]
 
     import torch
 
@@ -321,19 +379,12 @@ def main():
     jobs = []
     for b in range(B):
         rho_b = rho if b == 0 else fields.density_spiral_torus(g, seed=2 + b)
-        jobs.append(dict(grid=chypot.PreparedGrid(ctx, g), rho=chypot.to_device(rho_b),
-                         phi=torch.zeros((g.lmax, g.kmax + 2, g.jmax + 2), dtype=torch.float64,
-                                         device="cuda"),
-                         coef=torch.zeros((2, g.lmax // 2, g.kmax + 2, g.jmax + 2), dtype=torch.float64,
-                                          device="cuda"),
+        pg = chypot.PreparedGrid(ctx, g)
+        jobs.append(dict(grid=pg, rho=chypot.to_device(rho_b),
+                         phi=torch.zeros(pg.field_shape, dtype=torch.float64, device="cuda"),
+                         coef=torch.zeros(pg.coef_shape, dtype=torch.float64, device="cuda"),
                          coef_mode=chypot.COEF_K2))
     j0 = jobs[0]
-
-    def step():
-        ctx.solve_batch(jobs)
-
-    def single():
-        j0["grid"].solve(j0["rho"], j0["phi"], coef=j0["coef"], coef_mode=chypot.COEF_K2)
 
     if args.copy_mib > 0:
         src = torch.ones(args.copy_mib << 17, dtype=torch.float64, device="cuda")
@@ -342,16 +393,6 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-
-    # single-field passes (latency, stage times) run only when asked for, so
-    # that a profile of the batch alone sees only batched launches
-    n_lat = args.steps if args.latency_steps < 0 else args.latency_steps
-    n_st = args.steps if args.stage_steps < 0 else args.stage_steps
-    for _ in range(args.warmup):
-        step()
-        if n_lat > 0 or n_st > 0:
-            single()
-    torch.cuda.synchronize()
 
     def timed(fn, n):
         starts = [torch.cuda.Event(enable_timing=True) for _ in range(n)]
@@ -366,91 +407,108 @@ def main():
         t1 = time.perf_counter()
         return t1 - t0, np.array([a.elapsed_time(b) for a, b in zip(starts, ends)])
 
-    # timed region: batch steps (batched launches on a worker stream), with
-    # HIP events around every stage on the stream each launch runs on: the
-    # per-launch durations of the roofline come from the timed region itself
-    ctx.profile(True)
-    elapsed, lat_batch = timed(step, args.steps)
-    tstages = ctx.stage_times()
-    ctx.profile(False)
-    value = B * args.steps / elapsed
-    phi = chypot.to_host(j0["phi"])
-    # single-solve latency: one field per call on the context's stream
-    lat = timed(single, n_lat)[1] if n_lat > 0 else None
+    def set_solver(kind):
+        for j in jobs:
+            j["grid"].mode_solver = kind
 
-    # the refined path (binary32 correction solves on a binary64 residual):
-    # single-field solves of field 0, timed the same way, and its error
-    # against the plain binary64 result of the timed region (skipped with
-    # --refine-steps 0, e.g. to profile the plain path alone)
-    ref_phi = torch.zeros_like(j0["phi"])
-    ref_coef = torch.zeros_like(j0["coef"])
+    def batch_step(steps):
+        return lambda: ctx.solve_batch(jobs, refine_steps=steps)
 
-    def refined():
-        j0["grid"].solve_refined(j0["rho"], ref_phi, steps=args.refine_steps, coef=ref_coef,
-                                 coef_mode=chypot.COEF_K2)
+    def single(steps):
+        if steps:
+            return lambda: j0["grid"].solve_refined(j0["rho"], j0["phi"], steps=steps, coef=j0["coef"],
+                                                    coef_mode=chypot.COEF_K2)
+        return lambda: j0["grid"].solve(j0["rho"], j0["phi"], coef=j0["coef"], coef_mode=chypot.COEF_K2)
 
-    if args.refine_steps > 0:
-        for _ in range(args.warmup):
-            refined()
-        el_ref, lat_ref = timed(refined, args.steps)
-        _, rel_corr = j0["grid"].solve_refined(j0["rho"], ref_phi, steps=args.refine_steps, coef=ref_coef,
-                                               coef_mode=chypot.COEF_K2, report=True)
-        phi_ref = chypot.to_host(ref_phi)
-
-    # single-field steps launched kernel by kernel with HIP events around every
-    # stage on the solve's stream: the single-field stage times
-    stages = None
-    if n_st > 0:
-        ctx.profile(True)
-        for _ in range(n_st):
-            single()
-        stages = ctx.stage_times()
-        ctx.profile(False)
-
-    # accuracy of the timed result against the fixture solutions
+    n_lat = 2 * args.steps if args.latency_steps < 0 else args.latency_steps
     fx = np.load(ROOT / "tests" / "golden" / "solve_large_256x128x256.npz")
     scale = float(fx["phi_absmax"])
 
-    def err(a, b):
-        return float(np.abs(a - b).max() / scale)
+    def err_ref(phi):
+        return max(float(np.abs(phi[:, 1, :] - fx["phi_k2"]).max() / scale),
+                   float(np.abs(phi[:, :, 0] - fx["phi_l1"]).max() / scale))
 
-    e_ref = max(err(phi[:, 1, :], fx["phi_k2"]), err(phi[:, :, 0], fx["phi_l1"]))
-    refined_line = None if args.refine_steps <= 0 else {
-        "solves_per_s": args.steps / el_ref,
-        "p50_ms": float(np.percentile(lat_ref, 50)),
-        "p99_ms": float(np.percentile(lat_ref, 99)),
-        "steps": args.refine_steps,
-        "rel_correction_per_step": rel_corr,
-        "max_rel_err_vs_plain_fp64": float(np.abs(phi_ref - phi).max() / np.abs(phi).max()),
-        "max_rel_err_vs_reference_fp64": max(err(phi_ref[:, 1, :], fx["phi_k2"]),
-                                             err(phi_ref[:, :, 0], fx["phi_l1"])),
-    }
-    e_ext = max(err(phi[:, 1, :], fx["ext_phi_k2"]), err(phi[:, :, 0], fx["ext_phi_l1"]))
+    def run_path(label, solver, steps):
+        """warm-up, then the timed region (stage events on), then latency."""
+        set_solver(solver)
+        fn = batch_step(steps)
+        for _ in range(args.warmup):
+            fn()
+        ctx.profile(True)
+        elapsed, lat_batch = timed(fn, args.steps)
+        st = ctx.stage_times()
+        ctx.profile(False)
+        phi = chypot.to_host(j0["phi"])
+        res = {"solves_per_s": B * args.steps / elapsed, "ms_per_step": elapsed / args.steps * 1e3,
+               "batch_step_p50_ms": float(np.percentile(lat_batch, 50)),
+               "batch_step_p99_ms": float(np.percentile(lat_batch, 99)),
+               "max_rel_err_vs_reference_fp64": err_ref(phi)}
+        work = stage_work(g, "refined" if steps else ("cr" if solver == chypot.MODES_CR else "eigen"), steps)
+        res["rooflines"] = rooflines(st, work, B * args.steps, B, FP32_PEAK_TFLOPS if steps else FP64_PEAK_TFLOPS)
+        if n_lat > 0:
+            one = single(steps)
+            for _ in range(3):
+                one()
+            lat = timed(one, n_lat)[1]
+            res.update(p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                       single_solves_per_s=float(1e3 / np.mean(lat)))
+        return res, phi, elapsed
 
-    # roofline of the dominant stage of the timed region: its average launch
-    # duration and the fields one launch processes (B when the batch is one
-    # group of batched launches)
-    work = stage_work(g)
-    t_avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in tstages.items()}
-    dom = max(t_avg, key=lambda k: tstages[k][0])
-    avg_ms, launches = t_avg[dom], tstages[dom][1]
-    per_launch = B * args.steps / launches
-    wk = work[dom]
-    if wk["flops"] is not None and wk["flops"] / wk["bytes"] > FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9):
-        achieved = per_launch * wk["flops"] / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS}
-    else:
-        achieved = per_launch * (wk["bytes"] or 0) / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS}
-    roof.update({"traffic": hbm_traffic(dom, int(round(per_launch))), "kernel": dom, "avg_ms": avg_ms,
-                 "launches": launches, "fields_per_launch": per_launch})
-    hetero = hetero_batch(ctx, args.hetero) if args.hetero > 0 else None
+    out_paths = {}
+    phis = {}
+    elapsed_ref = None
+    if "cr" in paths:
+        out_paths["plain_fp64"], phis["cr"], _ = run_path("plain_fp64", chypot.MODES_CR, 0)
+        out_paths["plain_fp64"]["mode_solver"] = "cyclic reduction (BLKTRI's root tables)"
+    if "eigen" in paths:
+        out_paths["plain_fp64_eigen"], phis["eigen"], _ = run_path("eigen", chypot.MODES_EIGEN, 0)
+        out_paths["plain_fp64_eigen"]["mode_solver"] = "z eigenbasis (opt-in)"
+    set_solver(chypot.MODES_CR)
+    refined = None
+    if "refined" in paths:
+        refined, phis["refined"], elapsed_ref = run_path("refined", chypot.MODES_CR, args.refine_steps)
+        refined["refine_steps"] = args.refine_steps
+        refined["refine_m64"] = j0["grid"].refine_m64
+        # the error against the plain FP64 result, after each refinement step (field 0)
+        plain = phis.get("cr")
+        if plain is None:
+            j0["grid"].solve(j0["rho"], j0["phi"])
+            ctx.sync()
+            plain = chypot.to_host(j0["phi"])
+        pscale = float(np.abs(plain).max())
+        per_step = []
+        tmp = torch.zeros_like(j0["phi"])
+        for st in range(1, 5):
+            j0["grid"].solve_refined(j0["rho"], tmp, steps=st)
+            ctx.sync()
+            per_step.append(float(np.abs(chypot.to_host(tmp) - plain).max() / pscale))
+        refined["max_rel_err_vs_plain_fp64"] = float(np.abs(phis["refined"] - plain).max() / pscale)
+        refined["error_vs_plain_fp64_per_step"] = per_step
+
+    hetero = hetero_batch(ctx, args.hetero, args.refine_steps) if args.hetero > 0 else None
     serving = serve_run(ctx, args) if args.serve > 0 else None
+    ctx.set_stream(stream)
     source = (source_row(ctx, stream, g, args.source_calls, not args.no_cpu_baseline)
               if args.source_calls > 0 else None)
     cpu = None if args.no_cpu_baseline else cpu_baseline(g, rho, args.cpu_reps)
+
+    head = refined if refined is not None else out_paths.get("plain_fp64") or out_paths.get("plain_fp64_eigen")
+    value = head["solves_per_s"]
+    plain_v = out_paths.get("plain_fp64", {}).get("solves_per_s")
+    # roofline: the dominant stage of the headline timed region
+    rl = head["rooflines"]
+    dom = max(rl, key=lambda k: rl[k]["ms_total"])
+    d = rl[dom]
+    if "frac_mfma" in d and d.get("frac_mfma", 0) > d.get("frac_hbm", 0):
+        roof = {"bound": "mfma", "achieved": d["achieved_TFLOPs"],
+                "peak": FP32_PEAK_TFLOPS if refined is not None else FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": d["frac_mfma"]}
+    else:
+        roof = {"bound": "hbm", "achieved": d.get("achieved_GBs"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": d.get("frac_hbm")}
+    workload = "potsolve_256x128x256_refined" if refined is not None else "potsolve_256x128x256"
+    roof.update({"traffic": hbm_traffic(dom, B, workload), "kernel": dom, "avg_ms": d["ms_per_launch"],
+                 "launches": d["launches"], "fields_per_launch": B})
     out = {
         "metric": METRIC,
         "value": value,
@@ -458,29 +516,27 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if refined is not None else "f64",
         "data": "synthetic",
-        "config": {"workload": "plain FP64 potential solve (BDYGEN+POT3), spiral-arm torus",
-                   "grid": g.name, "path": "plain_fp64", "batch": B, "workers": W,
+        "config": {"workload": "refined potential solve (BDYGEN+POT3), batched, spiral-arm torus"
+                   if refined is not None else "plain FP64 potential solve (BDYGEN+POT3), batched, spiral-arm torus",
+                   "grid": g.name, "path": "refined" if refined is not None else "plain_fp64", "batch": B,
+                   "workers": W, "refine_steps": args.refine_steps if refined is not None else 0,
                    "coef": "K=2 plane", "maxtrm": 10, "isym": 2, "npoint": 8},
-        "p50_ms": None if lat is None else float(np.percentile(lat, 50)),
-        "p99_ms": None if lat is None else float(np.percentile(lat, 99)),
-        "single_solves_per_s": None if lat is None else float(1e3 / np.mean(lat)),
-        "batch_step_p50_ms": float(np.percentile(lat_batch, 50)),
-        "batch_step_p99_ms": float(np.percentile(lat_batch, 99)),
-        "max_rel_err_vs_reference_fp64": e_ref,
-        "max_rel_err_vs_exact": e_ext,
-        "refined_path": refined_line,
+        "vs_plain": None if (refined is None or not plain_v) else value / plain_v,
+        "vs_plain_eigen": None if (refined is None or "plain_fp64_eigen" not in out_paths)
+        else value / out_paths["plain_fp64_eigen"]["solves_per_s"],
+        "p50_ms": head.get("p50_ms"),
+        "p99_ms": head.get("p99_ms"),
+        "refined": refined,
+        **out_paths,
         "hetero_batch": hetero,
         "serving": serving,
         "source_grav": source,
-        "stages_ms": None if stages is None else {k: round(v[0] / v[1] if v[1] else 0.0, 5)
-                                                   for k, v in stages.items()},
-        "batch_stages_ms_per_launch": {k: round(v, 5) for k, v in t_avg.items() if tstages[k][1]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -25,7 +25,7 @@ LIB_PATH = _HERE.parent / "_lib" / "libchypot.so"
 
 CHY_OK, CHY_EINVAL, CHY_EUNSUPPORTED, CHY_ENOMEM, CHY_EDEVICE, CHY_ESTATE, CHY_ENUMERIC, CHY_ELIMIT = range(8)
 COEF_NONE, COEF_K2, COEF_FULL = 0, 1, 2
-STAGES = ("bdygen", "fft_fwd", "zfwd", "rsolve", "zinv", "fft_inv", "ghost", "fft_bdy", "cr")
+STAGES = ("bdygen", "fft_fwd", "zfwd", "rsolve", "zinv", "fft_inv", "ghost", "fft_bdy", "cr", "resid")
 # per-mode solver of a grid (chy_grid_set_mode_solver)
 MODES_CR, MODES_EIGEN = 0, 1
 # tunable kernel shapes and precision split (chy_grid_set_tune)

@@ -373,13 +373,20 @@ static int ensure_refine(chy_grid *g)
 static int rhs_spectrum_f(chy_grid *g, const Fields &F, hipStream_t s, int maxtrm, int isym, double redge,
                           int coef_mode)
 {
+    Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
     const bool spectral = bdygen_spectral_ok(g);
+    if (p) p->begin(CHY_STAGE_FFT_FWD, s);
     int rc = dev_rc(launch_fft_forward_f(g, F, coef_mode, LANES_INTERIOR, true, true, s), "forward FFT launch");
+    if (p) p->end(s);
+    if (p) p->begin(CHY_STAGE_BDYGEN, s);
     if (!rc) rc = dev_rc(launch_bdygen_from_moments_f(g, F, maxtrm, isym, redge, spectral, coef_mode, s),
                          "bdygen launch");
+    if (p) p->end(s);
     if (!rc && !spectral) {
+        if (p) p->begin(CHY_STAGE_FFT_BDY, s);
         rc = dev_rc(launch_fft_forward_f(g, F, coef_mode, LANES_BOUNDARY, false, true, s), "boundary FFT launch");
         if (!rc) rc = dev_rc(launch_rhs_edges_f(g, F, s), "boundary terms launch");
+        if (p) p->end(s);
     }
     return rc;
 }
@@ -415,17 +422,26 @@ static int run_refined_fields(chy_grid *g, const std::vector<chy_grid *> &grids,
         Fi.yh[f] = d.yh32;      // inverse: into X, the last step into spec
         Fi.aux[f] = d.yh;
     }
+    Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
     for (int it = 0; it < steps && !rc; ++it) {
         const bool last = it == steps - 1;
         const int epi = steps == 1 ? EPI_SPEC : (it == 0 ? EPI_XSET : (last ? EPI_XFIN : EPI_XADD));
-        if (it > 0) rc = dev_rc(launch_spec_residual_f(g, Fr, s), "spectral residual launch");
+        if (it > 0) {
+            if (p) p->begin(CHY_STAGE_RESID, s);
+            rc = dev_rc(launch_spec_residual_f(g, Fr, s), "spectral residual launch");
+            if (p) p->end(s);
+        }
         if (!rc)
             rc = dev_rc(launch_mode_solve32_f(g, it == 0 ? Ff0 : Ff, Fi, g->refine_m64, it == 0, epi,
-                                              with_stats ? it : -1, s),
+                                              with_stats ? it : -1, s, p),
                         "correction mode solve launch");
     }
+    if (p) p->begin(CHY_STAGE_FFT_INV, s);
     if (!rc) rc = dev_rc(launch_fft_inverse_f(g, F, s), "inverse FFT launch");
+    if (p) p->end(s);
+    if (p) p->begin(CHY_STAGE_GHOST, s);
     if (!rc) rc = dev_rc(launch_zaxphi_f(g, F, npoint, zaxphi_from_samples(g), s), "ZAXPHI launch");
+    if (p) p->end(s);
     return rc;
 }
 

@@ -313,8 +313,9 @@ int launch_mode_solve_f(const chy_grid *g, const Fields &F, bool rhs_edges, hipS
 // F.rstat + 2 * stat_step.
 enum { EPI_SPEC = 0, EPI_XSET = 1, EPI_XADD = 2, EPI_XFIN = 3 };
 int refine_supported(const chy_grid *g);
+struct Profiler;
 int launch_mode_solve32_f(const chy_grid *g, const Fields &Ff, const Fields &Fi, int m_fp64, bool rhs64, int epi,
-                          int stat_step, hipStream_t s);
+                          int stat_step, hipStream_t s, Profiler *prof);
 // Zero the first n words of every field's F.rstat (a fill kernel: callers may
 // capture the sequence into a graph).
 int launch_zero_u64_f(const Fields &F, int n, hipStream_t s);

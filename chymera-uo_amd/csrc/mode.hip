@@ -672,25 +672,31 @@ int launch_mode_solve(const chy_grid *g, bool rhs_edges, hipStream_t s, Profiler
 int refine_supported(const chy_grid *g) { return g->PP == 128 || g->PP == 64; }
 
 int launch_mode_solve32_f(const chy_grid *g, const Fields &Ff, const Fields &Fi, int m_fp64, bool rhs64, int epi,
-                          int stat_step, hipStream_t s)
+                          int stat_step, hipStream_t s, Profiler *prof)
 {
     if (!refine_supported(g)) return (int)hipErrorInvalidValue;
     ModeArgs a = make_args(g);
     const int m64 = std::max(-1, std::min(m_fp64, g->N));
+    if (prof) prof->begin(CHY_STAGE_ZFWD, s);
     if (rhs64)
         launch_zgemm<true, double, float, float>(g, a, Ff, s);
     else
         launch_zgemm<true, float, float, float>(g, a, Ff, s);
+    if (prof) prof->end(s);
+    if (prof) prof->begin(CHY_STAGE_RSOLVE, s);
     int rc = launch_rsolve<float, double>(g, a, Ff, s, 0, m64 + 1);
     if (!rc) rc = launch_rsolve<float, float>(g, a, Ff, s, m64 + 1, g->N + 1);
+    if (prof) prof->end(s);
     if (rc) return rc;
     a.stat_step = stat_step;
+    if (prof) prof->begin(CHY_STAGE_ZINV, s);
     switch (epi) {
     case EPI_XSET: launch_zgemm<false, float, double, float, EPI_XSET>(g, a, Fi, s); break;
     case EPI_XADD: launch_zgemm<false, float, double, float, EPI_XADD>(g, a, Fi, s); break;
     case EPI_XFIN: launch_zgemm<false, float, double, float, EPI_XFIN>(g, a, Fi, s); break;
     default: launch_zgemm<false, float, double, float, EPI_SPEC>(g, a, Fi, s); break;
     }
+    if (prof) prof->end(s);
     return (int)hipGetLastError();
 }
 

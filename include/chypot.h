@@ -325,13 +325,14 @@ int chy_find_force_noalloc(chy_ctx *ctx, const chy_force_desc *d, const double *
 enum {
     CHY_STAGE_BDYGEN = 0, /* radial split sums, boundary evaluation, ghosts */
     CHY_STAGE_FFT_FWD,    /* forward azimuthal RFFT (+ COEF, + BDYGEN ring moments) */
-    CHY_STAGE_ZFWD,       /* mode right-hand sides -> z eigen coordinates */
-    CHY_STAGE_RSOLVE,     /* r-tridiagonal solves */
-    CHY_STAGE_ZINV,       /* z eigen coordinates -> mode solutions */
+    CHY_STAGE_ZFWD,       /* mode right-hand sides -> z eigen coordinates (refined: every step's) */
+    CHY_STAGE_RSOLVE,     /* r-tridiagonal solves (refined: every step's) */
+    CHY_STAGE_ZINV,       /* z eigen coordinates -> mode solutions (refined: every step's) */
     CHY_STAGE_FFT_INV,    /* inverse azimuthal RFFT */
     CHY_STAGE_GHOST,      /* ghosts, ZAXPHI, PHICHK */
     CHY_STAGE_FFT_BDY,    /* forward RFFT of the boundary lanes (chy_solve) */
     CHY_STAGE_CR,         /* per-mode cyclic reduction (CHY_MODES_CR) */
+    CHY_STAGE_RESID,      /* refined solves: the mode systems' residual in the spectrum */
     CHY_NSTAGES
 };
 int chy_ctx_profile(chy_ctx *ctx, int enable);
