@@ -29,9 +29,10 @@ kernel for all of them):
     bytes per field (the scope table's figures: forward transform 134.2 MB,
     cyclic reduction 133.4 MB, inverse 135.8 MB at 256x128x256) over the
     stage time, against 8 TB/s; the z transforms also against the matrix-core
-    peak; `roofline` is the dominant stage of the headline (refined) timed
-    region, with `traffic` from the rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    of that kernel when profiles/ holds one;
+    peak, the cyclic reduction against the FP64 vector peak; where profiles/
+    holds a rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the workload, the HBM
+    bytes counted per launch beside the algorithmic ones; `roofline` is the
+    dominant stage of the headline (refined) timed region;
   * `hetero_batch` (64 fields of mixed grids, refined) and `serving` (Poisson
     arrivals over 60% 128x64x128, 30% 256x128x256, 10% 512x256x512 under an
     HBM budget) -- both on by default (--hetero 0 / --serve 0 skip them);
@@ -59,15 +60,37 @@ sys.path.insert(0, str(ROOT / "chymera-uo_amd"))
 
 METRIC = "pot3 solves/sec @256x128x256 refined vs plain FP64 at 1e-10; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
-FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 (matrix)
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 (matrix; the vector FP64 peak is the same)
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 (matrix)
+PEAKS = {"mfma_f64": FP64_PEAK_TFLOPS, "mfma_f32": FP32_PEAK_TFLOPS, "valu_f64": FP64_PEAK_TFLOPS}
+
+
+def cr_solves_per_mode(n):
+    """r solves of BLKTRI's cyclic reduction of one mode with n = 2^k - 1 z
+    rows, in the partial-fraction form the GPU runs: each reduction level's
+    odd rows apply a product with 2h - 1 divisor roots, each even row two
+    chains of h - 1 solves; each back-substitution row its two chains and a
+    product of 2h - 1 roots (pot/blktri.f:72-170)."""
+    kx = int(round(np.log2(n + 1)))
+    tot = 0
+    for lv in range(kx - 1):
+        h = 1 << lv
+        tot += len(range(h, n + 1, 2 * h)) * (2 * h - 1) + len(range(2 * h, n - h + 1, 2 * h)) * 2 * (h - 1)
+    for lv in range(kx - 1, -1, -1):
+        h, nrow = 1 << lv, (1 << (kx - lv)) - 1
+        for jj in range(1, nrow + 1, 2):
+            tot += (h - 1) * ((jj > 1) + (jj < nrow)) + 2 * h - 1
+    return tot
 
 
 def stage_work(g, path, steps):
-    """Algorithmic bytes (and matrix-core flops) per field of one solve, per
-    stage, summed over the stage's launches in that solve.  Transforms and
-    the cyclic reduction follow the scope table: forward 2 J K L 8 bytes,
-    reduction J K L 8 + M NZ L 8, inverse J K L 8 + (J+2)(K+2) L 8."""
+    """Algorithmic bytes (and flops, with the peak they are priced against)
+    per field of one solve, per stage, summed over the stage's launches in
+    that solve.  Transforms and the cyclic reduction follow the scope table:
+    forward 2 J K L 8 bytes, reduction J K L 8 + M NZ L 8, inverse J K L 8 +
+    (J+2)(K+2) L 8.  The reduction's flops: 10 per r row per solve (8 for a
+    tridiagonal elimination and back-substitution, 2 to combine), over the
+    solves of every mode system."""
     J, K, L = g.jmax, g.kmax, g.lmax
     M, NZ = J - 1, K - 1
     PP = ((NZ + 63) // 64) * 64
@@ -75,50 +98,58 @@ def stage_work(g, path, steps):
     lanes = J * K * L * 8
     modes = M * NZ * L * 8
     gemm = 2.0 * NZ * NZ * M * L
-    w = {"fft_fwd": dict(bytes=2 * lanes, flops=None),
-         "fft_inv": dict(bytes=lanes + (J + 2) * (K + 2) * L * 8, flops=None),
-         "bdygen": dict(bytes=None, flops=None), "ghost": dict(bytes=None, flops=None)}
+    none = dict(bytes=None, flops=None, peak=None)
+    w = {"fft_fwd": dict(bytes=2 * lanes, flops=None, peak=None),
+         "fft_inv": dict(bytes=lanes + (J + 2) * (K + 2) * L * 8, flops=None, peak=None),
+         "bdygen": none, "ghost": none}
     if path == "cr":
-        w["cr"] = dict(bytes=lanes + modes, flops=None)
+        w["cr"] = dict(bytes=lanes + modes, flops=10.0 * M * L * cr_solves_per_mode(NZ), peak="valu_f64")
     elif path == "eigen":
-        w["zfwd"] = dict(bytes=modes + PP * JP * L * 8, flops=gemm)
-        w["rsolve"] = dict(bytes=2 * modes, flops=None)
-        w["zinv"] = dict(bytes=PP * JP * L * 8 + modes, flops=gemm)
+        w["zfwd"] = dict(bytes=modes + PP * JP * L * 8, flops=gemm, peak="mfma_f64")
+        w["rsolve"] = dict(bytes=2 * modes, flops=None, peak=None)
+        w["zinv"] = dict(bytes=PP * JP * L * 8 + modes, flops=gemm, peak="mfma_f64")
     else:  # refined, binary32 mode solves; the first step reads the binary64 right-hand side
         m32 = modes // 2
-        w["zfwd"] = dict(bytes=modes + m32 + (steps - 1) * 2 * m32, flops=steps * gemm)
-        w["rsolve"] = dict(bytes=steps * 2 * m32, flops=None)
+        w["zfwd"] = dict(bytes=modes + m32 + (steps - 1) * 2 * m32, flops=steps * gemm, peak="mfma_f32")
+        w["rsolve"] = dict(bytes=steps * 2 * m32, flops=None, peak=None)
         xb = [m32 + modes] + [m32 + 2 * modes] * (steps - 1)  # X written, then read and written
-        w["zinv"] = dict(bytes=sum(xb), flops=steps * gemm)
-        w["resid"] = dict(bytes=(steps - 1) * (2 * modes + m32), flops=None)
+        w["zinv"] = dict(bytes=sum(xb), flops=steps * gemm, peak="mfma_f32")
+        w["resid"] = dict(bytes=(steps - 1) * (2 * modes + m32), flops=None, peak=None)
     return w
 
 
-# stage -> kernel whose rocprofv3 counters stand for it
-STAGE_KERNEL = {"fft_fwd": "k_fft_forward<16, 8, 32, double, double>",
-                "fft_inv": "k_fft_inverse<16, 8, 32, double, false>"}
+# stage -> name prefixes of the kernels its timed launches run
+STAGE_KERNELS = {"fft_fwd": ("k_fft_forward<",), "fft_inv": ("k_fft_inverse<",),
+                 "zfwd": ("k_zgemm_reg<true,",), "zinv": ("k_zgemm_reg<false,",),
+                 "rsolve": ("k_rsolve<",), "cr": ("k_cr_group<",), "resid": ("k_spec_residual",),
+                 "bdygen": ("k_bdy_",), "ghost": ("k_zaxphi<",)}
 
 
-def hbm_traffic(stage, batch, workload):
-    """HBM bytes per launch of the stage's kernel as rocprofv3 FETCH_SIZE +
-    WRITE_SIZE recorded them, each scaled by what the counter read for a
-    device-to-device copy of known size (profiles/<workload>_batch<B>_
-    hbm_traffic.txt), or None."""
+def hbm_traffic(stage, launches_per_step, batch, workload):
+    """HBM bytes per timed launch of a stage, from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE per dispatch of each of its kernels (each counter scaled by
+    what it read for a device-to-device copy of known size), weighted by the
+    kernel's dispatches per batch step: profiles/<workload>_batch<B>_
+    hbm_traffic.txt.  None without that summary."""
     f = ROOT / "profiles" / f"{workload}_batch{batch}_hbm_traffic.txt"
-    name = STAGE_KERNEL.get(stage)
-    if name is None or not f.exists():
+    pre = STAGE_KERNELS.get(stage)
+    if pre is None or not f.exists() or launches_per_step <= 0:
         return None
+    tot, seen = 0.0, False
     for line in f.read_text().splitlines():
         if line.startswith("kernel="):
             kv = dict(p.split("=", 1) for p in line.split("\t"))
-            if kv["kernel"] == name:
-                return float(kv["traffic_bytes"])
-    return None
+            if kv["kernel"].startswith(pre):
+                tot += float(kv["traffic_bytes"]) * float(kv["per_step"])
+                seen = True
+    return tot / launches_per_step if seen else None
 
 
-def rooflines(stage_times, work, nsolve, fields_per_launch, fp_peak):
-    """Per stage of a timed region of nsolve solves: total ms, launches, and
-    the algorithmic rate against the HBM (and matrix-core) peaks."""
+def rooflines(stage_times, work, nsolve, fields_per_launch, steps, workload):
+    """Per stage of a timed region of nsolve solves over `steps` batch steps:
+    total ms, launches, the algorithmic rate against the HBM peak (and the
+    compute peak the stage's flops are priced against), and the HBM bytes
+    rocprofv3 counted per launch (profiles/) beside the algorithmic ones."""
     out = {}
     for st, (ms, n) in stage_times.items():
         if not n or st not in work:
@@ -129,9 +160,14 @@ def rooflines(stage_times, work, nsolve, fields_per_launch, fp_peak):
         if w["bytes"]:
             gbs = w["bytes"] * nsolve / (ms * 1e-3) / 1e9
             e.update(bytes_per_field=w["bytes"], achieved_GBs=round(gbs, 1), frac_hbm=round(gbs / HBM_PEAK_GBS, 4))
+            tr = hbm_traffic(st, n / steps, fields_per_launch, workload)
+            if tr is not None:
+                alg = w["bytes"] * nsolve / n
+                e.update(traffic_bytes_per_launch=round(tr), traffic_vs_algorithmic=round(tr / alg, 3))
         if w["flops"]:
             tf = w["flops"] * nsolve / (ms * 1e-3) / 1e12
-            e.update(flops_per_field=w["flops"], achieved_TFLOPs=round(tf, 2), frac_mfma=round(tf / fp_peak, 4))
+            e.update(flops_per_field=w["flops"], achieved_TFLOPs=round(tf, 2), compute_peak=w["peak"],
+                     frac_compute=round(tf / PEAKS[w["peak"]], 4))
         out[st] = e
     return out
 
@@ -351,6 +387,9 @@ def main():
                     help="single-field solves per path for p50/p99 (default: 2 x --steps; 0 skips)")
     ap.add_argument("--source-calls", type=int, default=20,
                     help="also time this many calls of the SOURCE momentum sources (0: skip)")
+    ap.add_argument("--error-steps", type=int, default=4,
+                    help="refinement steps whose error against the plain FP64 result is reported "
+                         "(single-field solves after the timed region; 0 skips)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the solves, copy a buffer of this many MiB device to device "
                          "three times (a kernel of known traffic for scaling HBM counters)")
@@ -443,8 +482,9 @@ def main():
                "batch_step_p50_ms": float(np.percentile(lat_batch, 50)),
                "batch_step_p99_ms": float(np.percentile(lat_batch, 99)),
                "max_rel_err_vs_reference_fp64": err_ref(phi)}
-        work = stage_work(g, "refined" if steps else ("cr" if solver == chypot.MODES_CR else "eigen"), steps)
-        res["rooflines"] = rooflines(st, work, B * args.steps, B, FP32_PEAK_TFLOPS if steps else FP64_PEAK_TFLOPS)
+        kind = "refined" if steps else ("cr" if solver == chypot.MODES_CR else "eigen")
+        work = stage_work(g, kind, steps)
+        res["rooflines"] = rooflines(st, work, B * args.steps, B, args.steps, f"potsolve_{g.name}_{kind}")
         if n_lat > 0:
             one = single(steps)
             for _ in range(3):
@@ -471,19 +511,21 @@ def main():
         refined["refine_m64"] = j0["grid"].refine_m64
         # the error against the plain FP64 result, after each refinement step (field 0)
         plain = phis.get("cr")
-        if plain is None:
-            j0["grid"].solve(j0["rho"], j0["phi"])
+        if plain is None and args.error_steps > 0:
+            tmp = torch.zeros_like(j0["phi"])
+            j0["grid"].solve(j0["rho"], tmp)
             ctx.sync()
-            plain = chypot.to_host(j0["phi"])
-        pscale = float(np.abs(plain).max())
-        per_step = []
-        tmp = torch.zeros_like(j0["phi"])
-        for st in range(1, 5):
-            j0["grid"].solve_refined(j0["rho"], tmp, steps=st)
-            ctx.sync()
-            per_step.append(float(np.abs(chypot.to_host(tmp) - plain).max() / pscale))
-        refined["max_rel_err_vs_plain_fp64"] = float(np.abs(phis["refined"] - plain).max() / pscale)
-        refined["error_vs_plain_fp64_per_step"] = per_step
+            plain = chypot.to_host(tmp)
+        if plain is not None:
+            pscale = float(np.abs(plain).max())
+            refined["max_rel_err_vs_plain_fp64"] = float(np.abs(phis["refined"] - plain).max() / pscale)
+            per_step = []
+            tmp = torch.zeros_like(j0["phi"])
+            for st in range(1, args.error_steps + 1):
+                j0["grid"].solve_refined(j0["rho"], tmp, steps=st)
+                ctx.sync()
+                per_step.append(float(np.abs(chypot.to_host(tmp) - plain).max() / pscale))
+            refined["error_vs_plain_fp64_per_step"] = per_step
 
     hetero = hetero_batch(ctx, args.hetero, args.refine_steps) if args.hetero > 0 else None
     serving = serve_run(ctx, args) if args.serve > 0 else None
@@ -499,16 +541,15 @@ def main():
     rl = head["rooflines"]
     dom = max(rl, key=lambda k: rl[k]["ms_total"])
     d = rl[dom]
-    if "frac_mfma" in d and d.get("frac_mfma", 0) > d.get("frac_hbm", 0):
-        roof = {"bound": "mfma", "achieved": d["achieved_TFLOPs"],
-                "peak": FP32_PEAK_TFLOPS if refined is not None else FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": d["frac_mfma"]}
+    if d.get("frac_compute", 0) > d.get("frac_hbm", 0):
+        roof = {"bound": "mfma" if d["compute_peak"].startswith("mfma") else "valu", "achieved": d["achieved_TFLOPs"],
+                "peak": PEAKS[d["compute_peak"]], "unit": "TFLOP/s", "frac": d["frac_compute"]}
     else:
         roof = {"bound": "hbm", "achieved": d.get("achieved_GBs"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": d.get("frac_hbm")}
-    workload = "potsolve_256x128x256_refined" if refined is not None else "potsolve_256x128x256"
-    roof.update({"traffic": hbm_traffic(dom, B, workload), "kernel": dom, "avg_ms": d["ms_per_launch"],
-                 "launches": d["launches"], "fields_per_launch": B})
+    roof.update({"traffic": d.get("traffic_bytes_per_launch"), "algorithmic_bytes_per_launch":
+                 None if not d.get("bytes_per_field") else round(d["bytes_per_field"] * B * args.steps / d["launches"]),
+                 "kernel": dom, "avg_ms": d["ms_per_launch"], "launches": d["launches"], "fields_per_launch": B})
     out = {
         "metric": METRIC,
         "value": value,
