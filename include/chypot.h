@@ -107,7 +107,8 @@ size_t chy_grid_device_bytes(const chy_grid *g);
  *   CHY_MODES_CR    (default) BLKTRI's generalised cyclic reduction
  *                   (pot/blktri.f:60-260) with BLKTRI's own root tables
  *                   (COMPB/TQLRAT, pot/blktri.f:264-423, the same arithmetic):
- *                   the reference's results to rounding (~1e-14 relative).
+ *                   the reference's results to rounding (2.2e-13 relative at
+ *                   256x128x256).
  *   CHY_MODES_EIGEN the m-independent z operator diagonalised once per grid
  *                   (80-bit arithmetic) and r-tridiagonal solves per z
  *                   eigenvalue, the transforms on FP64 matrix cores: closer to
@@ -205,17 +206,19 @@ int chy_pot3_host(chy_grid *g, int npoint, const double *rho_h, double *phi_h, d
 
 /* ---- refined solve (no reference counterpart) ----
  * The same results as chy_solve (PHI, COEF, diagnostics) to a stated
- * accuracy, by iterative refinement: BDYGEN and COEF in binary64 as
- * chy_solve computes them, then `steps` (1..8) correction steps -- the
- * binary64 residual of the discrete 7-point operator the mode solves invert,
- * rounded to binary32; its azimuthal transform (binary32 in and out); the
- * mode solves in binary32 (z transforms on the FP32 matrix cores, r solves
- * in binary32 except the orders m <= CHY_TUNE_REFINE_M64, which are solved in
- * binary64); the inverse transform added to PHI in binary64 -- then POT3's
- * ghost cells and ZAXPHI.  The first step starts from 0 in the interior
- * (whatever phi_d held there is not read).  rel_corr (host, may be NULL)
- * receives, per step, the largest correction relative to the largest |PHI|
- * after it (1 for the first step).  Waits when diag or rel_corr is non-NULL. */
+ * accuracy, by iterative refinement of the mode systems in the azimuthal
+ * spectrum: BDYGEN, COEF and the forward transform in binary64 as chy_solve
+ * computes them; then `steps` (1..8) mode solves in binary32 (z transforms
+ * on the FP32 matrix cores, r solves in binary32 except the orders
+ * m <= CHY_TUNE_REFINE_M64, which are solved in binary64), the first of the
+ * right-hand side, each later one of the binary64 residual of the mode
+ * systems (pot/pot3.f:203-231) at the binary64 solution so far, rounded to
+ * binary32, its solution added in binary64; then the inverse transform,
+ * POT3's ghost cells and ZAXPHI as in chy_solve.  Two steps put PHI within
+ * ~1e-11 of chy_solve's at 256x128x256.  KMAX <= 128 (CHY_EUNSUPPORTED
+ * otherwise).  rel_corr (host, may be NULL) receives, per step, the largest
+ * correction relative to the largest |solution| after it (1 for the first
+ * step).  Waits when diag or rel_corr is non-NULL. */
 int chy_solve_refined(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho_d,
                       double *phi_d, double *coef_d, int coef_mode, int steps, chy_diag *diag,
                       double *rel_corr);

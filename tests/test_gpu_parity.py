@@ -274,8 +274,17 @@ def test_config2_eigenbasis_against_exact(ctx):
 
 @pytest.mark.parametrize("jkl,code", [((16, 6, 16), chypot.CHY_EUNSUPPORTED),
                                       ((16, 8, 12), chypot.CHY_EUNSUPPORTED),
+                                      ((16, 8, 96), chypot.CHY_EUNSUPPORTED),
+                                      ((16, 8, 2048), chypot.CHY_EUNSUPPORTED),
+                                      ((1040, 8, 16), chypot.CHY_EUNSUPPORTED),
                                       ((2, 8, 16), chypot.CHY_EINVAL)])
 def test_rejected_grids(ctx, jkl, code):
+    """Grids the drop-in refuses with a code instead of solving wrongly:
+    KMAX not 2^k (BLKTRI's own precondition, pot/blktri.f:21-31); LMAX not a
+    power of two <= 1024, although the reference's FFT also takes LMAX/2 with
+    prime factors up to 23 (pot/fft.f:111-126; 96 = 2 x 48 is such a case):
+    the transforms here are radix-2 kernels specialised per size; JMAX above
+    1024, where the r solves would need more than 16 rows per lane; JMAX < 3."""
     g = fields.unit_grid(*jkl)
     with pytest.raises(chypot.ChyError) as ei:
         chypot.PreparedGrid(ctx, g)
