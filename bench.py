@@ -108,6 +108,9 @@ def stage_work(g, path, steps):
         w["zfwd"] = dict(bytes=modes + PP * JP * L * 8, flops=gemm, peak="mfma_f64")
         w["rsolve"] = dict(bytes=2 * modes, flops=None, peak=None)
         w["zinv"] = dict(bytes=PP * JP * L * 8 + modes, flops=gemm, peak="mfma_f64")
+    elif path == "refined_fused":  # one kernel per mode plane (1 or 2 steps)
+        w["planes"] = (dict(bytes=modes * 2, flops=2 * gemm, peak="mfma_f32") if steps == 1 else
+                       dict(bytes=2 * modes + modes + modes, flops=4 * gemm, peak="mfma_f32"))
     else:  # refined, binary32 mode solves; the first step reads the binary64 right-hand side
         m32 = modes // 2
         w["zfwd"] = dict(bytes=modes + m32 + (steps - 1) * 2 * m32, flops=steps * gemm, peak="mfma_f32")
@@ -122,6 +125,7 @@ def stage_work(g, path, steps):
 STAGE_KERNELS = {"fft_fwd": ("k_fft_forward<",), "fft_inv": ("k_fft_inverse<",),
                  "zfwd": ("k_zgemm_reg<true,",), "zinv": ("k_zgemm_reg<false,",),
                  "rsolve": ("k_rsolve<",), "cr": ("k_cr_group<",), "resid": ("k_spec_residual",),
+                 "planes": ("k_refine_plane<",),
                  "bdygen": ("k_bdy_",), "ghost": ("k_zaxphi<",)}
 
 
@@ -373,7 +377,7 @@ def main():
     ap.add_argument("--batch", type=int, default=32, help="independent fields per step")
     ap.add_argument("--workers", type=int, default=0, help="worker streams (default min(batch, 8))")
     ap.add_argument("--refine-steps", type=int, default=2, help="correction steps of the refined solve")
-    ap.add_argument("--paths", default="refined,cr,eigen",
+    ap.add_argument("--paths", default="refined,cr,eigen,refined_fused",
                     help="which paths to time (refined is the headline; a profile may time one)")
     ap.add_argument("--cpu-reps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -483,6 +487,8 @@ def main():
                "batch_step_p99_ms": float(np.percentile(lat_batch, 99)),
                "max_rel_err_vs_reference_fp64": err_ref(phi)}
         kind = "refined" if steps else ("cr" if solver == chypot.MODES_CR else "eigen")
+        if steps and st.get("planes", (0, 0))[1]:
+            kind = "refined_fused"
         work = stage_work(g, kind, steps)
         res["rooflines"] = rooflines(st, work, B * args.steps, B, args.steps, f"potsolve_{g.name}_{kind}")
         if n_lat > 0:
@@ -526,6 +532,18 @@ def main():
                 ctx.sync()
                 per_step.append(float(np.abs(chypot.to_host(tmp) - plain).max() / pscale))
             refined["error_vs_plain_fp64_per_step"] = per_step
+    if "refined_fused" in paths:
+        # the same refined solve with all mode work of a plane in one kernel
+        # (binary32 plane in LDS, one HBM round trip): the comparison row
+        for j in jobs:
+            j["grid"].refine_fused = True
+        fused, phi_f, _ = run_path("refined_fused", chypot.MODES_CR, args.refine_steps)
+        for j in jobs:
+            j["grid"].refine_fused = False
+        ref_phi = phis.get("refined")
+        if ref_phi is not None:
+            fused["max_rel_diff_vs_refined"] = float(np.abs(phi_f - ref_phi).max() / np.abs(ref_phi).max())
+        out_paths["refined_one_kernel_per_plane"] = fused
 
     hetero = hetero_batch(ctx, args.hetero, args.refine_steps) if args.hetero > 0 else None
     serving = serve_run(ctx, args) if args.serve > 0 else None
@@ -534,7 +552,8 @@ def main():
               if args.source_calls > 0 else None)
     cpu = None if args.no_cpu_baseline else cpu_baseline(g, rho, args.cpu_reps)
 
-    head = refined if refined is not None else out_paths.get("plain_fp64") or out_paths.get("plain_fp64_eigen")
+    head = (refined if refined is not None else out_paths.get("plain_fp64") or out_paths.get("plain_fp64_eigen")
+            or out_paths.get("refined_one_kernel_per_plane"))
     value = head["solves_per_s"]
     plain_v = out_paths.get("plain_fp64", {}).get("solves_per_s")
     # roofline: the dominant stage of the headline timed region

@@ -25,11 +25,11 @@ LIB_PATH = _HERE.parent / "_lib" / "libchypot.so"
 
 CHY_OK, CHY_EINVAL, CHY_EUNSUPPORTED, CHY_ENOMEM, CHY_EDEVICE, CHY_ESTATE, CHY_ENUMERIC, CHY_ELIMIT = range(8)
 COEF_NONE, COEF_K2, COEF_FULL = 0, 1, 2
-STAGES = ("bdygen", "fft_fwd", "zfwd", "rsolve", "zinv", "fft_inv", "ghost", "fft_bdy", "cr", "resid")
+STAGES = ("bdygen", "fft_fwd", "zfwd", "rsolve", "zinv", "fft_inv", "ghost", "fft_bdy", "cr", "resid", "planes")
 # per-mode solver of a grid (chy_grid_set_mode_solver)
 MODES_CR, MODES_EIGEN = 0, 1
 # tunable kernel shapes and precision split (chy_grid_set_tune)
-TUNE_CR_LANES, TUNE_REFINE_M64 = 0, 1
+TUNE_CR_LANES, TUNE_REFINE_M64, TUNE_REFINE_FUSED = 0, 1, 2
 
 # every entry point declared in include/chypot.h
 EXPORTS = (
@@ -339,6 +339,15 @@ class PreparedGrid:
     @refine_m64.setter
     def refine_m64(self, m: int):
         _check(lib().chy_grid_set_tune(self.handle, TUNE_REFINE_M64, int(m)), "chy_grid_set_tune")
+
+    @property
+    def refine_fused(self) -> bool:
+        """Refined solves of 1 or 2 steps: all mode work of a plane in one kernel."""
+        return bool(lib().chy_grid_tune(self.handle, TUNE_REFINE_FUSED))
+
+    @refine_fused.setter
+    def refine_fused(self, on: bool):
+        _check(lib().chy_grid_set_tune(self.handle, TUNE_REFINE_FUSED, int(bool(on))), "chy_grid_set_tune")
 
     # device tensor shapes: PHI/RHO (LMAX, KMAX+2, JMAX+2); COEF (2, LMAX/2, KMAX+2, JMAX+2)
     @property

@@ -363,6 +363,10 @@ static int ensure_refine(chy_grid *g)
     int rc = get((void **)&g->d.spec32, sizeof(float) * (size_t)g->L * g->K * g->J);
     if (!rc) rc = get((void **)&g->d.yh32, sizeof(float) * (size_t)g->L * g->PP * g->JP);
     if (!rc) rc = get((void **)&g->d.rstat, sizeof(unsigned long long) * 2 * kMaxRefine);
+    if (!rc && !g->d.fac32) {
+        rc = get((void **)&g->d.fac32, sizeof(float) * 2 * (size_t)(g->N + 1) * g->PP * g->JP);
+        if (!rc) rc = dev_rc(launch_thomas_factors(g, g->ctx->stream), "r-solve factor launch");
+    }
     return rc;
 }
 
@@ -423,6 +427,16 @@ static int run_refined_fields(chy_grid *g, const std::vector<chy_grid *> &grids,
         Fi.aux[f] = d.yh;
     }
     Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
+    // the fused per-plane kernel where it applies (it collects no per-step
+    // statistics: those calls take the kernel-per-stage path)
+    if (g->refine_fused && !with_stats && refine_planes_ok(g, steps)) {
+        Fields Fp = F;
+        for (int f = 0; f < F.nf; ++f) Fp.aux[f] = grids[f]->d.yh32;
+        if (p) p->begin(CHY_STAGE_PLANES, s);
+        if (!rc) rc = dev_rc(launch_refine_planes_f(g, Fp, steps, g->refine_m64, s), "refined mode planes launch");
+        if (p) p->end(s);
+        steps = 0;  // the mode work is done
+    }
     for (int it = 0; it < steps && !rc; ++it) {
         const bool last = it == steps - 1;
         const int epi = steps == 1 ? EPI_SPEC : (it == 0 ? EPI_XSET : (last ? EPI_XFIN : EPI_XADD));
@@ -829,6 +843,7 @@ int chy_grid_set_tune(chy_grid *g, int param, int value)
     int *slot = nullptr;
     if (param == CHY_TUNE_CR_LANES && cr_lanes_ok(g, value)) slot = &g->cr_lanes;
     if (param == CHY_TUNE_REFINE_M64 && value >= -1 && value <= g->N) slot = &g->refine_m64;
+    if (param == CHY_TUNE_REFINE_FUSED && (value == 0 || value == 1)) slot = &g->refine_fused;
     if (!slot) return CHY_EINVAL;
     if (*slot == value) return CHY_OK;
     for (auto &e : g->graphs) retire_graph(e);
@@ -842,6 +857,7 @@ int chy_grid_tune(const chy_grid *g, int param)
     if (!g) return -1;
     if (param == CHY_TUNE_CR_LANES) return g->cr_lanes;
     if (param == CHY_TUNE_REFINE_M64) return g->refine_m64;
+    if (param == CHY_TUNE_REFINE_FUSED) return g->refine_fused;
     return -1;
 }
 

@@ -91,6 +91,9 @@ struct DeviceGrid {
     double *wf = nullptr;        // [NZ][PP]   Y -> eigen coordinates:  Yh[p] = sum_k wf[k][p] Y[k]
     double *wi = nullptr;        // [NZ][PP]   eigen coordinates -> X:  X[k] = sum_p wi[p][k] Z[p]
     double *lam = nullptr;       // [PP]       eigenvalues (padded entries 1)
+    float *wf32 = nullptr;       // [PP][PP]   wf rounded to binary32, zero past NZ (rows k, columns p)
+    float *wi32 = nullptr;       // [PP][PP]   wi rounded to binary32, zero past NZ (rows p, columns k)
+    float *fac32 = nullptr;      // [N+1][JP][PP] x 2: LU factors of T_m + lambda_p (refined solves, first use)
     // r operator (pot3.f:172-194,226-230): AM(1)=0, CM(M)=0, BM per azimuthal
     // order m; rows padded to JP (rows >= M: AM = CM = 0, BM = 1)
     double *am = nullptr;        // [JP]
@@ -224,6 +227,7 @@ struct chy_grid {
     int mode_solver = CHY_MODES_CR;  // CHY_MODES_CR or CHY_MODES_EIGEN
     int cr_lanes = 64;               // lanes per r system in the cyclic reduction (64 or 32)
     int refine_m64 = 4;              // refinement: orders m <= this solved in binary64
+    int refine_fused = 0;            // refinement: one kernel per mode plane where the grid allows it
     size_t device_bytes = 0;
     std::vector<void *> allocations;
     chy::DeviceGrid d;
@@ -313,6 +317,15 @@ int launch_mode_solve_f(const chy_grid *g, const Fields &F, bool rhs_edges, hipS
 // F.rstat + 2 * stat_step.
 enum { EPI_SPEC = 0, EPI_XSET = 1, EPI_XADD = 2, EPI_XFIN = 3 };
 int refine_supported(const chy_grid *g);
+// The whole mode work of a 1- or 2-step refined solve in one kernel per mode
+// plane (binary32 plane in LDS): reads the binary64 right-hand side spectrum
+// F.spec, writes the solution into it; F.aux: a binary32 scratch of
+// L x PP x JP per field (two steps).  refine_planes_ok: the plane fits the
+// kernel (KMAX <= 129, JMAX <= 257) and steps is 1 or 2.
+bool refine_planes_ok(const chy_grid *g, int steps);
+// fills d.fac32 (allocated by the caller) on stream s
+int launch_thomas_factors(const chy_grid *g, hipStream_t s);
+int launch_refine_planes_f(const chy_grid *g, const Fields &F, int steps, int m_fp64, hipStream_t s);
 struct Profiler;
 int launch_mode_solve32_f(const chy_grid *g, const Fields &Ff, const Fields &Fi, int m_fp64, bool rhs64, int epi,
                           int stat_step, hipStream_t s, Profiler *prof);

@@ -348,6 +348,12 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
     }
 }
 
+// a / b in arithmetic AT: the binary64 division for binary64; for binary32
+// (the refinement's correction solves, whose errors the next step removes)
+// the product with the hardware reciprocal (1 ulp)
+__device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // One row of the reduced (per-lane) tridiagonal system, in arithmetic AT.
 template <typename AT>
 struct Red {
@@ -454,7 +460,7 @@ __device__ __forceinline__ void rsolve_rows(const ModeArgs &a, int m, int p, int
 #pragma unroll
         for (int i = 0; i < R - 1; ++i) {
             const AT den = i == 0 ? cb[0] : cb[i] - ca[i] * g[i - 1];
-            const AT inv = (AT)1 / den;
+            const AT inv = qdiv((AT)1, den);
             g[i] = cc[i] * inv;
             const AT rp = i == 0 ? -ca[0] : (AT)0;
             const AT rq = i == R - 2 ? -cc[R - 2] : (AT)0;
@@ -489,8 +495,8 @@ __device__ __forceinline__ void rsolve_rows(const ModeArgs &a, int m, int p, int
     for (int s = 1; s < 64; s <<= 1) {
         const Red<AT> lo = shfl_up_red(red, s, lane);
         const Red<AT> hi = shfl_down_red(red, s, lane);
-        const AT k1 = red.al / lo.be;
-        const AT k2 = red.ga / hi.be;
+        const AT k1 = qdiv(red.al, lo.be);
+        const AT k2 = qdiv(red.ga, hi.be);
         Red<AT> nr;
         nr.al = -lo.al * k1;
         nr.ga = -hi.ga * k2;
@@ -499,7 +505,7 @@ __device__ __forceinline__ void rsolve_rows(const ModeArgs &a, int m, int p, int
         nr.d1 = red.d1 - lo.d1 * k1 - hi.d1 * k2;
         red = nr;
     }
-    const AT xr0 = red.d0 / red.be, xr1 = red.d1 / red.be;
+    const AT xr0 = qdiv(red.d0, red.be), xr1 = qdiv(red.d1, red.be);
 
     // back-substitution into the lane's own rows
     x0[R - 1] = xr0;
@@ -758,6 +764,481 @@ __global__ __launch_bounds__(256) void k_spec_residual(ModeArgs a, const Fields 
     }
 }
 
+namespace {
+
+// ---------------------------------------------------------------------------
+// The refined solve's mode work for one mode plane in one kernel.
+//
+// One block per plane q of one field.
+// The binary32 plane, PP rows x JP columns, lives in LDS for the whole plane
+// and is rewritten in place by each phase; nothing but the right-hand side B
+// (read twice) and the result X (written once) goes through HBM:
+//   1. z forward of B (binary64 from the spectrum, in 32-row chunks that are
+//      loaded while the previous chunk is multiplied):  Y = W_f^T B
+//   2. r solves (T_m + lambda_p) y_p = Y_p: one lane per row p with the
+//      grid's LU factors (binary64 solves of the orders m <= m64: one
+//      wavefront per row, parallel cyclic reduction)
+//   3. z inverse X1 = W_i Y (binary32)
+// then, for two steps:
+//   4. the binary64 residual R = B - A X1 of the plane's system, its stencil
+//      read from X1 staged in LDS, rounded to binary32 (X1 also goes to a
+//      binary32 scratch plane, F.aux, [q][PP][JP])
+//   5. 1..3 again on R: the correction C
+//   6. X = X1 + C in binary64 into the spectrum.
+// With one step, X1 is the result.  HBM traffic per plane: B twice, X1 in
+// binary32 out and back, X out.  The products run on the FP32 matrix
+// cores (V_MFMA_F32_16X16X4F32): wave w owns output rows 16w..16w+15 and
+// all JP columns, A operands from the binary32 weight tables, B operands
+// from the LDS plane.  The r solves of the orders m <=
+// m64 run in binary64 arithmetic, the others in binary32.  An LDS row is JP +
+// 17 floats: the 64 lanes of a per-lane r solve read one column from 64
+// banks, and a B operand's four rows meet at most two to a bank.  A block
+// needs PP (JP + 17) 4 bytes of LDS: 140 KB at PP = 128, JP = 256.
+
+// One r-tridiagonal system over the 64 lanes of a wavefront, lane i owning
+// rows [iR, iR+R): cb already holds the diagonal (1 on padding rows, where
+// d = 0); partitioned elimination in the lane, then parallel cyclic
+// reduction of the lanes' interface rows.
+template <int R, typename AT>
+__device__ __forceinline__ void rsolve1(const AT (&ca)[R], const AT (&cb)[R], const AT (&cc)[R], int lane,
+                                        const AT (&d)[R], AT (&x)[R])
+{
+    AT D[R], P[R], Q[R];
+    AT al, be, ga, dd;
+    if constexpr (R == 1) {
+        al = ca[0]; be = cb[0]; ga = cc[0]; dd = d[0];
+    } else {
+        AT g[R], w[R], wp[R], wq[R];
+#pragma unroll
+        for (int i = 0; i < R - 1; ++i) {
+            const AT den = i == 0 ? cb[0] : cb[i] - ca[i] * g[i - 1];
+            const AT inv = qdiv((AT)1, den);
+            g[i] = cc[i] * inv;
+            const AT rp = i == 0 ? -ca[0] : (AT)0;
+            const AT rq = i == R - 2 ? -cc[R - 2] : (AT)0;
+            w[i] = (d[i] - (i ? ca[i] * w[i - 1] : (AT)0)) * inv;
+            wp[i] = (rp - (i ? ca[i] * wp[i - 1] : (AT)0)) * inv;
+            wq[i] = (rq - (i ? ca[i] * wq[i - 1] : (AT)0)) * inv;
+        }
+        D[R - 2] = w[R - 2]; P[R - 2] = wp[R - 2]; Q[R - 2] = wq[R - 2];
+#pragma unroll
+        for (int i = R - 3; i >= 0; --i) {
+            D[i] = w[i] - g[i] * D[i + 1];
+            P[i] = wp[i] - g[i] * P[i + 1];
+            Q[i] = wq[i] - g[i] * Q[i + 1];
+        }
+        const AT nD = __shfl_down(D[0], 1, 64), nP = __shfl_down(P[0], 1, 64), nQ = __shfl_down(Q[0], 1, 64);
+        const bool last = lane == 63;
+        const AT c = last ? (AT)0 : cc[R - 1];
+        al = ca[R - 1] * P[R - 2];
+        be = cb[R - 1] + ca[R - 1] * Q[R - 2] + c * (last ? (AT)0 : nP);
+        ga = c * (last ? (AT)0 : nQ);
+        dd = d[R - 1] - ca[R - 1] * D[R - 2] - c * (last ? (AT)0 : nD);
+    }
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        AT lal = __shfl_up(al, s, 64), lbe = __shfl_up(be, s, 64), lga = __shfl_up(ga, s, 64);
+        AT ld = __shfl_up(dd, s, 64);
+        AT hal = __shfl_down(al, s, 64), hbe = __shfl_down(be, s, 64), hga = __shfl_down(ga, s, 64);
+        AT hd = __shfl_down(dd, s, 64);
+        if (lane < s) { lal = (AT)0; lbe = (AT)1; lga = (AT)0; ld = (AT)0; }
+        if (lane + s >= 64) { hal = (AT)0; hbe = (AT)1; hga = (AT)0; hd = (AT)0; }
+        const AT k1 = qdiv(al, lbe), k2 = qdiv(ga, hbe);
+        const AT nal = -lal * k1, nga = -hga * k2;
+        const AT nbe = be - lga * k1 - hal * k2;
+        const AT nd = dd - ld * k1 - hd * k2;
+        al = nal; ga = nga; be = nbe; dd = nd;
+    }
+    const AT xr = qdiv(dd, be);
+    x[R - 1] = xr;
+    if constexpr (R > 1) {
+        AT xl = __shfl_up(xr, 1, 64);
+        if (lane == 0) xl = (AT)0;
+#pragma unroll
+        for (int i = 0; i < R - 1; ++i) x[i] = D[i] + P[i] * xl + Q[i] * xr;
+    }
+}
+
+// The r solves of one plane in LDS (rows p < NZ), NS systems at a time per
+// wavefront (independent chains, so each one's shuffle and reciprocal
+// latency hides behind the others'); arithmetic AT.
+template <int R, int LP, typename AT, int NS>
+__device__ __forceinline__ void plane_rsolve(const ModeArgs &a, float *S, int m, int w, int lane)
+{
+    const int f = lane * R;
+    AT ca[R], cc[R], cb0[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        ca[r] = (AT)a.am[f + r];
+        cc[r] = (AT)a.cm[f + r];
+        cb0[r] = (AT)a.bm[(size_t)m * a.JP + f + r];
+    }
+#pragma unroll 1
+    for (int i = 0; i < 16; i += NS) {
+        const int p0 = 16 * w + i;
+        if (p0 >= a.NZ) break;  // rows past NZ are zero and stay zero (wave-uniform)
+        AT d[NS][R], x[NS][R], cb[NS][R];
+#pragma unroll
+        for (int t = 0; t < NS; ++t) {
+            const bool on = p0 + t < a.NZ;
+            const AT lam = on ? (AT)a.lam[p0 + t] : (AT)0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                d[t][r] = on ? (AT)S[(p0 + t) * LP + f + r] : (AT)0;
+                cb[t][r] = f + r < a.M ? cb0[r] + lam : (AT)1;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NS; ++t) rsolve1<R, AT>(ca, cb[t], cc, lane, d[t], x[t]);
+#pragma unroll
+        for (int t = 0; t < NS; ++t)
+            if (p0 + t < a.NZ)
+#pragma unroll
+                for (int r = 0; r < R; ++r) S[(p0 + t) * LP + f + r] = (float)x[t][r];
+    }
+}
+
+// The binary32 r solves of one plane in LDS by one lane per system: lane l
+// of wavefront w solves row p = 64 w + l (p < NZ) with the LU factors of
+// T_m + lambda_p prepared once per grid, fac = (l_j, 1/pivot_j) per row j:
+//   forward   D_j = d_j - l_j D_{j-1}
+//   backward  x_j = (D_j - c_j x_{j+1}) / pivot_j
+// over all JP rows (rows past M carry l = 0, 1/pivot = 1, c = 0, d = 0).
+// Each step's chain is one FMA; the other products and every load are off
+// the chain, and the loads of the next TB rows are issued before the current
+// TB rows are computed.  The sweeps need no cross-lane traffic; the row
+// pitch LP = JP + 17 puts the 64 lanes' elements of one column in 64
+// different banks.  cS: c_j (CM) in LDS.
+template <int LP, int JP>
+__device__ __forceinline__ void plane_thomas(float *S, const float *cS, const float *fac, int PPg, int NZ, int w,
+                                             int lane)
+{
+    const int p = 64 * w + lane;
+    if (p >= NZ) return;
+    float *row = S + p * LP;
+    const float2 *fr = reinterpret_cast<const float2 *>(fac) + p;  // row j at fr[j PP]
+    constexpr int TB = 16, NB = JP / TB;
+    float2 fc[TB], fn[TB];
+    float dc[TB], dn[TB];
+    auto load = [&](float2 (&f)[TB], float (&d)[TB], int j0) {
+#pragma unroll
+        for (int i = 0; i < TB; ++i) {
+            f[i] = fr[(size_t)(j0 + i) * PPg];
+            d[i] = row[j0 + i];
+        }
+    };
+    float D = 0.0f;
+    load(fc, dc, 0);
+#pragma unroll 1
+    for (int b = 0; b < NB; ++b) {
+        if (b + 1 < NB) load(fn, dn, (b + 1) * TB);
+#pragma unroll
+        for (int i = 0; i < TB; ++i) {
+            D = __builtin_fmaf(-fc[i].x, D, dc[i]);
+            row[b * TB + i] = D;
+        }
+#pragma unroll
+        for (int i = 0; i < TB; ++i) { fc[i] = fn[i]; dc[i] = dn[i]; }
+    }
+    float x = 0.0f;
+    load(fc, dc, JP - TB);
+#pragma unroll 1
+    for (int b = NB - 1; b >= 0; --b) {
+        if (b > 0) load(fn, dn, (b - 1) * TB);
+#pragma unroll
+        for (int i = TB - 1; i >= 0; --i) {
+            const int j = b * TB + i;
+            const float inv = fc[i].y;
+            x = __builtin_fmaf(-cS[j] * inv, x, dc[i] * inv);
+            row[j] = x;
+        }
+#pragma unroll
+        for (int i = 0; i < TB; ++i) { fc[i] = fn[i]; dc[i] = dn[i]; }
+    }
+}
+
+// The LU factors of T_m + lambda_p for every order m <= N and z eigenvalue
+// p < NZ, binary64 arithmetic rounded once: fac[((m JP + j) PP + p)] =
+// (l_j, 1/pivot_j) with pivot_0 = b_0, l_j = a_j / pivot_{j-1}, pivot_j =
+// b_j - l_j c_{j-1}, b_j = BM_m(j) + lambda_p; rows j >= M: (0, 1).  One
+// thread per (m, p); p fastest, so the lanes of a per-lane solve read one
+// row j of 64 systems together.
+__global__ __launch_bounds__(256) void k_thomas_factors(ModeArgs a, float *fac)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (a.N + 1) * a.NZ) return;
+    const int m = t / a.NZ, p = t % a.NZ;
+    const double lam = a.lam[p];
+    float2 *f = reinterpret_cast<float2 *>(fac) + (size_t)m * a.JP * a.PP + p;
+    double piv = 1.0;
+    for (int j = 0; j < a.JP; ++j) {
+        if (j >= a.M) {
+            f[(size_t)j * a.PP] = make_float2(0.0f, 1.0f);
+            continue;
+        }
+        const double l = j > 0 ? a.am[j] / piv : 0.0;
+        piv = a.bm[(size_t)m * a.JP + j] + lam - (j > 0 ? l * a.cm[j - 1] : 0.0);
+        f[(size_t)j * a.PP] = make_float2((float)l, (float)(1.0 / piv));
+    }
+}
+
+// Launch bounds: at most 256 registers per lane for both plane heights, so
+// the products accumulate in VGPRs.  The PP = 64 build with a 512-register
+// budget kept accumulators in AGPRs and its second step came out wrong (1e-6
+// off the kernel-per-stage result, at -O3 only, while -O1 matched it to
+// 1e-15); the VGPR-only build matches to rounding.
+template <int PP, int JP>
+__global__ __launch_bounds__(PP * 4, 128 / PP) void k_refine_plane(ModeArgs a, const Fields F, const double *bn,
+                                                           const float *wf32, const float *wi32, const float *fac,
+                                                           int steps, int m64, int nplanes)
+{
+    typedef Mfma<float> MM;
+    typedef typename MM::acc acc_t;
+    constexpr int NT = PP * 4;    // threads: PP / 16 wavefronts
+    constexpr int U = JP / 16;    // 16-column MFMA tiles per wavefront
+    constexpr int KS = PP / 4;    // k-steps of the 16x16x4 product
+    constexpr int R = JP / 64;    // r rows per lane
+    constexpr int LP = JP + 17;   // LDS row pitch (floats)
+    constexpr int CH = 32;        // rows per chunk of the first forward product
+    constexpr int CPER = CH * JP / NT;  // chunk elements per thread
+    static_assert(CPER * NT == CH * JP, "chunk split");
+    __shared__ float S[PP * LP];
+    __shared__ float cS[JP];  // CM (zero past M)
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6, lr = lane & 15, lk = lane >> 4;
+    const int NZ = a.NZ, M = a.M;
+
+    // products over the plane in LDS: acc[u] += sum_ks A[16w+lr][4ks+lk] B[4ks+lk][16u+lr]
+    // for ks0 <= ks < ks1.  A comes from the binary32 weight table W32
+    // ([red][row], PP x PP, zero past NZ: rows p >= NZ of Y and k >= NZ of X
+    // come out zero) one k-step ahead, the B operands of the next k-step are
+    // read from LDS while the current one is multiplied.
+    auto product = [&](const float *W32, acc_t (&acc)[U], int ks0, int ks1) {
+        const float *wp = W32 + (size_t)lk * PP + 16 * w + lr;
+        float an = wp[(size_t)(4 * ks0) * PP];
+        float bn_[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) bn_[u] = S[(4 * ks0 + lk) * LP + 16 * u + lr];
+#pragma unroll 1
+        for (int ks = ks0; ks < ks1; ++ks) {
+            const float ac = an;
+            float bc[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) bc[u] = bn_[u];
+            if (ks + 1 < ks1) {
+                an = wp[(size_t)(4 * (ks + 1)) * PP];
+#pragma unroll
+                for (int u = 0; u < U; ++u) bn_[u] = S[(4 * (ks + 1) + lk) * LP + 16 * u + lr];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc[u] = MM::mac(ac, bc[u], acc[u]);
+        }
+    };
+    // accumulators -> LDS rows 16w + 4lk + v, columns 16u + lr
+    auto put = [&](const acc_t (&acc)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) S[(16 * w + MM::row(lane, v)) * LP + 16 * u + lr] = acc[u][v];
+    };
+    auto rsolve = [&](int m) {
+        if (m <= m64) plane_rsolve<R, LP, double, 2>(a, S, m, w, lane);
+        else if (w < PP / 64) plane_thomas<LP, JP>(S, cS, fac + (size_t)2 * m * PP * JP, a.PP, NZ, w, lane);
+    };
+
+    {
+        // XCD-aware plane order: the blocks of one XCD (block b runs on XCD
+        // b mod 8) take a contiguous run of planes sorted by order m, fields
+        // fastest, so the planes sharing m's r factors meet in one L2
+        const int nf = nplanes / a.L;
+        const int chunk = (nplanes + 7) / 8;
+        const int sidx = (blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
+        if (sidx >= nplanes) return;  // the whole block leaves together
+        const int qs = sidx / nf, fi = sidx % nf;
+        // qs: 0 -> m 0; 2m-1 -> cosine of m; 2m -> sine of m (m < N); L-1 -> m N
+        const int q = qs == 0 ? 0 : (qs == a.L - 1 ? a.N : ((qs & 1) ? (qs + 1) / 2 : a.N + qs / 2));
+        const int m = q <= a.N ? q : q - a.N;
+        double *spec = static_cast<double *>(F.spec[fi]) + (size_t)q * a.K * a.J;
+        for (int j = tid; j < JP; j += NT) cS[j] = (float)a.cm[j];  // read after the barriers below
+
+        // 1. z forward of B, chunk by chunk
+        acc_t acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u] = MM::zero();
+        double cb[CPER];
+        auto fetch = [&](int c) {
+#pragma unroll
+            for (int i = 0; i < CPER; ++i) {
+                const int e = tid + NT * i, k = c * CH + e / JP, j = e % JP;
+                const int kc = k < NZ ? k : NZ - 1, jc = j < M ? j : M - 1;
+                cb[i] = spec[(size_t)kc * a.J + jc];
+            }
+        };
+        auto stash = [&](int c) {
+#pragma unroll
+            for (int i = 0; i < CPER; ++i) {
+                const int e = tid + NT * i, k = c * CH + e / JP, j = e % JP;
+                S[k * LP + j] = (k < NZ && j < M) ? (float)cb[i] : 0.0f;
+            }
+        };
+        fetch(0);
+        stash(0);
+        __syncthreads();
+        constexpr int NCH = PP / CH;
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            if (c + 1 < NCH) fetch(c + 1);
+            product(wf32, acc, c * (CH / 4), (c + 1) * (CH / 4));
+            if (c + 1 < NCH) stash(c + 1);
+            __syncthreads();
+        }
+        put(acc);
+        __syncthreads();
+
+        // 2. r solves
+        rsolve(m);
+        __syncthreads();
+
+        // 3. z inverse
+        acc_t x1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) x1[u] = MM::zero();
+        product(wi32, x1, 0, KS);
+
+        if (steps == 1) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = 16 * w + MM::row(lane, v), j = 16 * u + lr;
+                    if (k < NZ && j < M) spec[(size_t)k * a.J + j] = (double)x1[u][v];
+                }
+            return;
+        }
+
+        // X1 to the binary32 scratch plane (read back for the final sum) and
+        // to LDS (the residual's stencil)
+        float *xs = static_cast<float *>(F.aux[fi]) + (size_t)q * PP * JP;
+        __syncthreads();  // every wavefront has read Y
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * w + MM::row(lane, v), j = 16 * u + lr;
+                S[r * LP + j] = x1[u][v];
+                xs[(size_t)r * JP + j] = x1[u][v];
+            }
+        __syncthreads();
+
+        // 4. residual of the plane's system at X1, binary64, rounded to
+        // binary32; cells past NZ or M are zero
+        {
+            float rr[U][4];
+            double an4[4], bd4[4], cn4[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int k = 16 * w + MM::row(lane, v), kc = k < NZ ? k : NZ - 1;
+                an4[v] = a.an_dev[kc];
+                bd4[v] = bn[kc];
+                cn4[v] = a.cn_dev[kc];
+            }
+            // B of UG column groups at a time, the next batch loaded while
+            // the current one is used
+            constexpr int UG = U < 2 ? U : 2;
+            double bcur[UG][4], bnxt[UG][4];
+            auto load_b = [&](double (&bb)[UG][4], int u0) {
+#pragma unroll
+                for (int g = 0; g < UG; ++g) {
+                    const int j = 16 * (u0 + g) + lr, jc = j < M ? j : M - 1;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int k = 16 * w + MM::row(lane, v), kc = k < NZ ? k : NZ - 1;
+                        bb[g][v] = spec[(size_t)kc * a.J + jc];
+                    }
+                }
+            };
+            load_b(bcur, 0);
+#pragma unroll
+            for (int u0 = 0; u0 < U; u0 += UG) {
+                if (u0 + UG < U) load_b(bnxt, u0 + UG);
+#pragma unroll
+                for (int g = 0; g < UG; ++g) {
+                    const int u = u0 + g;
+                    const int j = 16 * u + lr, jc = j < M ? j : M - 1;
+                    const double amj = a.am[jc], cmj = a.cm[jc], bmj = a.bm[(size_t)m * a.JP + jc];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        // every read in bounds (clamped), the out-of-system
+                        // neighbours weighted by zero: no branches
+                        const int k = 16 * w + MM::row(lane, v);
+                        const int km = k > 0 ? k - 1 : k, kp = k + 1 < PP ? k + 1 : k;
+                        const int jm = j > 0 ? j - 1 : j;
+                        const float sx = S[k * LP + j], sm = S[km * LP + j], sp = S[kp * LP + j];
+                        const float sl = S[k * LP + jm], sr = S[k * LP + j + 1];
+                        const double xm = k > 0 ? (double)sm : 0.0;
+                        const double xp = k + 1 < NZ ? (double)sp : 0.0;
+                        const double xl = j > 0 ? (double)sl : 0.0;
+                        const double xr = j + 1 < M ? (double)sr : 0.0;
+                        const double ax =
+                            an4[v] * xm + (bd4[v] + bmj) * (double)sx + cn4[v] * xp + amj * xl + cmj * xr;
+                        rr[u][v] = (k < NZ && j < M) ? (float)(bcur[g][v] - ax) : 0.0f;
+                    }
+                }
+                // one batch of B loads in flight at a time
+                __builtin_amdgcn_sched_barrier(0);
+                if (u0 + UG < U) {
+#pragma unroll
+                    for (int g = 0; g < UG; ++g)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) bcur[g][v] = bnxt[g][v];
+                }
+            }
+            __syncthreads();  // every wavefront has read X1's stencil
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) S[(16 * w + MM::row(lane, v)) * LP + 16 * u + lr] = rr[u][v];
+            __syncthreads();
+        }
+
+        // 5. the correction: z forward of R, r solves, z inverse
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u] = MM::zero();
+        product(wf32, acc, 0, KS);
+        __syncthreads();
+        put(acc);
+        __syncthreads();
+        rsolve(m);
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u] = MM::zero();
+        product(wi32, acc, 0, KS);
+
+        // 6. X = X1 + C in binary64 into the spectrum (this wavefront wrote
+        // the X1 values it reads back)
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int k = 16 * w + MM::row(lane, v), j = 16 * u + lr;
+                if (k < NZ && j < M)
+                    spec[(size_t)k * a.J + j] = (double)xs[(size_t)k * JP + j] + (double)acc[u][v];
+            }
+    }
+}
+
+template <int PP, int JP>
+void launch_refine_plane(const chy_grid *g, const ModeArgs &a, const Fields &F, int steps, int m64, hipStream_t s)
+{
+    const int nplanes = g->L * F.nf;
+    const int blocks = 8 * ((nplanes + 7) / 8);
+    hipLaunchKernelGGL((k_refine_plane<PP, JP>), dim3(blocks), dim3(PP * 4), 0, s, a, F, g->d.rz_bn, g->d.wf32,
+                       g->d.wi32, g->d.fac32, steps, m64, nplanes);
+}
+
+}  // namespace
+
 int launch_spec_residual_f(const chy_grid *g, const Fields &F, hipStream_t s)
 {
     ModeArgs a = make_args(g);
@@ -767,5 +1248,42 @@ int launch_spec_residual_f(const chy_grid *g, const Fields &F, hipStream_t s)
     hipLaunchKernelGGL(k_spec_residual, grid, dim3(256), 0, s, a, F, g->d.rz_bn);
     return (int)hipGetLastError();
 }
+
+int launch_thomas_factors(const chy_grid *g, hipStream_t s)
+{
+    const ModeArgs a = make_args(g);
+    const int n = (g->N + 1) * g->NZ;
+    hipLaunchKernelGGL(k_thomas_factors, dim3((n + 255) / 256), dim3(256), 0, s, a, g->d.fac32);
+    return (int)hipGetLastError();
+}
+
+bool refine_planes_ok(const chy_grid *g, int steps)
+{
+    return (steps == 1 || steps == 2) && (g->PP == 64 || g->PP == 128) && g->JP <= 256 && g->NZ >= 2;
+}
+
+int launch_refine_planes_f(const chy_grid *g, const Fields &F, int steps, int m_fp64, hipStream_t s)
+{
+    if (!refine_planes_ok(g, steps)) return (int)hipErrorInvalidValue;
+    ModeArgs a = make_args(g);
+    a.an_dev = g->d.rz_an;
+    a.cn_dev = g->d.rz_cn;
+    const int m64 = std::max(-1, std::min(m_fp64, g->N));
+    if (g->PP == 128) {
+        switch (g->JP) {
+        case 64: launch_refine_plane<128, 64>(g, a, F, steps, m64, s); break;
+        case 128: launch_refine_plane<128, 128>(g, a, F, steps, m64, s); break;
+        default: launch_refine_plane<128, 256>(g, a, F, steps, m64, s); break;
+        }
+    } else {
+        switch (g->JP) {
+        case 64: launch_refine_plane<64, 64>(g, a, F, steps, m64, s); break;
+        case 128: launch_refine_plane<64, 128>(g, a, F, steps, m64, s); break;
+        default: launch_refine_plane<64, 256>(g, a, F, steps, m64, s); break;
+        }
+    }
+    return (int)hipGetLastError();
+}
+
 
 }  // namespace chy

@@ -440,6 +440,18 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
     CHY_TRY(upload(g, &d.wi, wi));
     CHY_TRY(upload(g, &d.lam, lam));
     {
+        // binary32 copies of the weight tables for the refined solve's fused
+        // plane kernel, PP x PP with zero rows past NZ
+        std::vector<float> wf32((size_t)PP * PP, 0.0f), wi32((size_t)PP * PP, 0.0f);
+        for (int r = 0; r < NZ; ++r)
+            for (int c = 0; c < PP; ++c) {
+                wf32[(size_t)r * PP + c] = (float)wf[(size_t)r * PP + c];
+                wi32[(size_t)r * PP + c] = (float)wi[(size_t)r * PP + c];
+            }
+        CHY_TRY(upload(g, &d.wf32, wf32));
+        CHY_TRY(upload(g, &d.wi32, wi32));
+    }
+    {
         // r coefficients padded to JP rows per order (rows >= M: AM = CM = 0,
         // BM = 1) so that the r solves read whole aligned rows
         const int JP = g->JP;

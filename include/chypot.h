@@ -129,10 +129,17 @@ int chy_grid_mode_solver(const chy_grid *g);
  *   CHY_TUNE_REFINE_M64  refined solves: the r solves of orders m <= this
  *                        value run in binary64, the others in binary32
  *                        (-1..LMAX/2; default 4)
+ *   CHY_TUNE_REFINE_FUSED refined solves of 1 or 2 steps: 1 runs all mode
+ *                        work of a mode plane in one kernel with the binary32
+ *                        plane in LDS where it fits (KMAX <= 129, JMAX <=
+ *                        257): a third of the HBM traffic, but one block per
+ *                        CU runs its phases one after another, and at
+ *                        256x128x256 it is no faster; 0 (default) one kernel
+ *                        per stage
  * Results do not depend on the shapes beyond rounding.  Changing a value
  * drops the grid's captured solve graphs.  CHY_EINVAL for a value the grid
  * cannot take. */
-enum { CHY_TUNE_CR_LANES = 0, CHY_TUNE_REFINE_M64 = 1 };
+enum { CHY_TUNE_CR_LANES = 0, CHY_TUNE_REFINE_M64 = 1, CHY_TUNE_REFINE_FUSED = 2 };
 int chy_grid_set_tune(chy_grid *g, int param, int value);
 int chy_grid_tune(const chy_grid *g, int param);
 
@@ -336,6 +343,7 @@ enum {
     CHY_STAGE_FFT_BDY,    /* forward RFFT of the boundary lanes (chy_solve) */
     CHY_STAGE_CR,         /* per-mode cyclic reduction (CHY_MODES_CR) */
     CHY_STAGE_RESID,      /* refined solves: the mode systems' residual in the spectrum */
+    CHY_STAGE_PLANES,     /* refined solves: every step's mode work, one kernel per mode plane */
     CHY_NSTAGES
 };
 int chy_ctx_profile(chy_ctx *ctx, int enable);

@@ -55,6 +55,27 @@ def test_refined_matches_reference(ctx, name):
     assert np.abs(phase_diff(ck[..., 1], rk[..., 1])[sig]).max() < 1e-6
 
 
+@pytest.mark.parametrize("name", SMALL_CASES)
+def test_refined_two_steps_matches_reference(ctx, name):
+    """Two steps by the one-kernel-per-plane form against the reference's
+    whole solve; the interior of PHI starts as NaN."""
+    g, rho, params, f = load_case(name)
+    pg = chypot.PreparedGrid(ctx, g, isym=params["isym"])
+    pg.refine_fused = True
+    rho_d = chypot.to_device(rho)
+    phi = torch.full_like(rho_d, float("nan"))
+    pg.solve_refined(rho_d, phi, steps=2, maxtrm=params["maxtrm"], isym=params["isym"], redge=params["redge"],
+                     npoint=params["npoint"])
+    ctx.sync()
+    pg.close()
+    phi = chypot.to_host(phi)
+    phi[:, g.kmax + 1, :] = 0.0          # cells POT3 leaves alone (NaN here)
+    phi[g.jmax + 1, 0, :] = 0.0
+    err = rel_max(phi, f["phi"])
+    print(f"[gpu] refined 2 steps {name}: vs reference {err:.2e}", flush=True)
+    assert err < 1e-10
+
+
 @pytest.mark.parametrize("steps", [2, 3])
 def test_refined_256_against_plain_and_reference(ctx, steps):
     """The benchmark grid: the refined solve against the plain binary64 GPU
@@ -113,6 +134,36 @@ def test_refined_error_per_step_256(ctx, density):
     assert errs[0] < 1e-5
     assert errs[1] < 1e-10
     assert max(errs[2:]) < 1e-11
+
+
+@pytest.mark.parametrize("jkl", [(256, 128, 256), (64, 32, 64), (100, 64, 32), (200, 128, 64), (40, 16, 16),
+                                 (257, 8, 16)])
+@pytest.mark.parametrize("steps", [1, 2])
+def test_fused_planes_match_stage_kernels(ctx, jkl, steps):
+    """The one-kernel-per-mode-plane form of the refined solve (the binary32
+    plane in LDS) against the kernel-per-stage form on the same grid: the
+    same products and residual; the binary32 r solves differ in method (one
+    lane per system with the grid's LU factors, against parallel cyclic
+    reduction across a wavefront), which moves the ~1e-7 first step, so one
+    step agrees to that and two steps to rounding.  Shapes: every (PP, JP)
+    the fused kernel is built for, JMAX-1 = JP (the r system filling every
+    lane's rows) and KMAX = 8."""
+    g = fields.unit_grid(*jkl)
+    rho_d = chypot.to_device(fields.density_spiral_torus(g, seed=21))
+    pg = chypot.PreparedGrid(ctx, g)
+    out = {}
+    for fused in (True, False):
+        pg.refine_fused = fused
+        assert pg.refine_fused == fused
+        phi = torch.zeros_like(rho_d)
+        pg.solve_refined(rho_d, phi, steps=steps)
+        ctx.sync()
+        out[fused] = phi
+    pg.close()
+    scale = float(out[False].abs().max())
+    err = float((out[True] - out[False]).abs().max()) / scale
+    print(f"[gpu] fused planes {g.name} steps {steps}: vs stage kernels {err:.2e}", flush=True)
+    assert err < (1e-5 if steps == 1 else 2e-11)
 
 
 def test_batched_refined_equals_single(ctx):
