@@ -209,7 +209,7 @@ def hetero_batch(ctx, n, refine_steps, reps=3):
     return out
 
 
-def serve_run(ctx, args):
+def serve_run(ctx, args, tune_table=None):
     """Config 5: the serving loop (chypot.serve) over Poisson arrivals,
     refined where the grid allows it: throughput and per-request latency
     from arrival to completion."""
@@ -221,7 +221,7 @@ def serve_run(ctx, args):
     torch.cuda.synchronize()
     reqs = serve.poisson_requests(args.serve_rate, args.serve, mix, dens, seed=5)
     srv = serve.Server(ctx, hbm_budget=int(args.serve_budget_gib * (1 << 30)), max_batch=16, workers=8,
-                       refine_steps=args.refine_steps)
+                       refine_steps=args.refine_steps, tune_table=tune_table)
     srv.prepare([shape for shape, _ in mix], per_shape=4)
     t0 = time.perf_counter()
     done = srv.run(reqs)
@@ -394,6 +394,8 @@ def main():
     ap.add_argument("--error-steps", type=int, default=4,
                     help="refinement steps whose error against the plain FP64 result is reported "
                          "(single-field solves after the timed region; 0 skips)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the library's default launch shapes instead of timing the candidates first")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the solves, copy a buffer of this many MiB device to device "
                          "three times (a kernel of known traffic for scaling HBM counters)")
@@ -428,6 +430,16 @@ def main():
                          coef=torch.zeros(pg.coef_shape, dtype=torch.float64, device="cuda"),
                          coef_mode=chypot.COEF_K2))
     j0 = jobs[0]
+    tune_table = None
+    if args.autotune:
+        # launch shapes of the bench grid for single fields and for the batch
+        from chypot import autotune
+        t_tune = time.perf_counter()
+        tune_table = autotune.tune(ctx, [g], batch_sizes=(1, B), refine_steps=args.refine_steps, reps=3)
+        for j in jobs:
+            autotune.apply(j["grid"], tune_table)
+        ctx.set_stream(stream)
+        tune_table[autotune.grid_key(g)]["seconds"] = round(time.perf_counter() - t_tune, 2)
 
     if args.copy_mib > 0:
         src = torch.ones(args.copy_mib << 17, dtype=torch.float64, device="cuda")
@@ -535,18 +547,19 @@ def main():
     if "refined_fused" in paths:
         # the same refined solve with all mode work of a plane in one kernel
         # (binary32 plane in LDS, one HBM round trip): the comparison row
+        tuned_fused = j0["grid"].refine_fused
         for j in jobs:
             j["grid"].refine_fused = True
         fused, phi_f, _ = run_path("refined_fused", chypot.MODES_CR, args.refine_steps)
         for j in jobs:
-            j["grid"].refine_fused = False
+            j["grid"].refine_fused = tuned_fused
         ref_phi = phis.get("refined")
         if ref_phi is not None:
             fused["max_rel_diff_vs_refined"] = float(np.abs(phi_f - ref_phi).max() / np.abs(ref_phi).max())
         out_paths["refined_one_kernel_per_plane"] = fused
 
     hetero = hetero_batch(ctx, args.hetero, args.refine_steps) if args.hetero > 0 else None
-    serving = serve_run(ctx, args) if args.serve > 0 else None
+    serving = serve_run(ctx, args, tune_table) if args.serve > 0 else None
     ctx.set_stream(stream)
     source = (source_row(ctx, stream, g, args.source_calls, not args.no_cpu_baseline)
               if args.source_calls > 0 else None)
@@ -598,6 +611,7 @@ def main():
         "serving": serving,
         "source_grav": source,
         "roofline": roof,
+        "autotune": tune_table,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

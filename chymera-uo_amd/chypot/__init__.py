@@ -341,13 +341,16 @@ class PreparedGrid:
         _check(lib().chy_grid_set_tune(self.handle, TUNE_REFINE_M64, int(m)), "chy_grid_set_tune")
 
     @property
-    def refine_fused(self) -> bool:
-        """Refined solves of 1 or 2 steps: all mode work of a plane in one kernel."""
-        return bool(lib().chy_grid_tune(self.handle, TUNE_REFINE_FUSED))
+    def refine_fused(self) -> int:
+        """Refined solves of 1 or 2 steps launched for at most this many
+        fields run all mode work of a plane in one kernel (0: never).  True
+        and False set 32 and 0."""
+        return int(lib().chy_grid_tune(self.handle, TUNE_REFINE_FUSED))
 
     @refine_fused.setter
-    def refine_fused(self, on: bool):
-        _check(lib().chy_grid_set_tune(self.handle, TUNE_REFINE_FUSED, int(bool(on))), "chy_grid_set_tune")
+    def refine_fused(self, n):
+        n = 32 if n is True else (0 if n is False else int(n))
+        _check(lib().chy_grid_set_tune(self.handle, TUNE_REFINE_FUSED, n), "chy_grid_set_tune")
 
     # device tensor shapes: PHI/RHO (LMAX, KMAX+2, JMAX+2); COEF (2, LMAX/2, KMAX+2, JMAX+2)
     @property

@@ -22,6 +22,8 @@ a grid shape and a device-resident density; the server
 refine_steps > 0 serves the refined solve (binary32 correction solves on the
 FP32 matrix cores, binary64 residuals) on grids it supports (KMAX <= 128);
 other grids get the plain binary64 solve.  Each request records its path.
+A chypot.autotune table (tune_table) sets the launch shapes of every grid
+the server prepares.
 
 Solves are deterministic and independent, so a request's potential does not
 depend on what it was batched with (tests/test_gpu_serve.py checks this bit
@@ -37,7 +39,7 @@ import time
 
 import numpy as np
 
-from . import COEF_K2, Context, PreparedGrid, fields, to_device
+from . import COEF_K2, Context, PreparedGrid, autotune, fields, to_device
 
 
 @dataclasses.dataclass
@@ -82,7 +84,8 @@ class Slot:
 
 class Server:
     def __init__(self, ctx: Context, hbm_budget: int, max_batch: int = 16, workers: int = 4,
-                 keep_outputs: bool = False, refine_steps: int = 0, slots_per_shape: int = 16):
+                 keep_outputs: bool = False, refine_steps: int = 0, slots_per_shape: int = 16,
+                 tune_table: dict | None = None):
         import torch
         self.torch = torch
         self.ctx = ctx
@@ -91,6 +94,7 @@ class Server:
         self.keep_outputs = keep_outputs  # copy each result out of its slot at completion
         self.refine_steps = int(refine_steps)
         self.slots_per_shape = slots_per_shape
+        self.tune_table = tune_table or {}  # chypot.autotune table applied to every new slot's grid
         ctx.workers(workers)
         # eager launches: each request brings its own density buffer, so a
         # captured single-field graph (keyed on its pointers) would rarely replay
@@ -147,6 +151,8 @@ class Server:
         if not self._evict_until(need, shape):
             return None
         s = Slot(self.ctx, shape, self.torch)
+        if self.tune_table:
+            autotune.apply(s.grid, self.tune_table)
         s.bytes = s.grid.device_bytes + field_bytes(shape) + 2 * (shape[2] // 2) * (shape[1] + 2) * (shape[0] + 2) * 8
         if self._refined(shape):
             s.bytes += field_bytes(shape)  # the binary32 spectrum and eigen coordinates (allocated on first use)

@@ -22,6 +22,11 @@ void set_device_error(hipError_t e, const char *where)
     g_last_error = std::string(where) + ": " + hipGetErrorString(e);
 }
 
+// a captured group's key: kGroupKeyHead argument words, then per field its
+// grid, density, PHI, COEF and KLOCAT
+constexpr size_t kGroupKeyHead = 8, kGroupKeyPerField = 5;
+constexpr size_t kGroupGraphCache = 16;
+
 static int dev_rc(int hip_rc, const char *where)
 {
     if (hip_rc == (int)hipSuccess) return CHY_OK;
@@ -193,6 +198,34 @@ static void retire_graph(chy_grid::GraphEntry &e)
     if (e.exec) (void)hipGraphExecDestroy(e.exec);
     e.done = nullptr;
     e.exec = nullptr;
+}
+
+static void retire_group_graph(chy_ctx::GroupGraph &e)
+{
+    if (e.done) {
+        (void)hipEventSynchronize(e.done);
+        (void)hipEventDestroy(e.done);
+    }
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    e.done = nullptr;
+    e.exec = nullptr;
+}
+
+// drops the captured group graphs that launch work on grid g (all with g null)
+static void drop_group_graphs(chy_ctx *c, const chy_grid *g)
+{
+    auto &v = c->group_graphs;
+    for (size_t i = 0; i < v.size();) {
+        bool hit = g == nullptr;
+        for (size_t k = kGroupKeyHead; !hit && k < v[i].key.size(); k += kGroupKeyPerField)
+            hit = v[i].key[k] == (uintptr_t)g;
+        if (hit) {
+            retire_group_graph(v[i]);
+            v.erase(v.begin() + (long)i);
+        } else {
+            ++i;
+        }
+    }
 }
 
 // BDYGEN alone: ring moments from a transform of the interior lanes (no
@@ -429,7 +462,7 @@ static int run_refined_fields(chy_grid *g, const std::vector<chy_grid *> &grids,
     Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
     // the fused per-plane kernel where it applies (it collects no per-step
     // statistics: those calls take the kernel-per-stage path)
-    if (g->refine_fused && !with_stats && refine_planes_ok(g, steps)) {
+    if (F.nf <= g->refine_fused && !with_stats && refine_planes_ok(g, steps)) {
         Fields Fp = F;
         for (int f = 0; f < F.nf; ++f) Fp.aux[f] = grids[f]->d.yh32;
         if (p) p->begin(CHY_STAGE_PLANES, s);
@@ -581,6 +614,7 @@ int chy_ctx_destroy(chy_ctx *ctx)
     DeviceScope ds(ctx->device);
     ctx->prof.collect();
     ctx->prof.release();
+    drop_group_graphs(ctx, nullptr);
     release_workers(ctx);
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->d_diag) (void)hipFree(ctx->d_diag);
@@ -609,6 +643,71 @@ int chy_ctx_workers(chy_ctx *ctx, int n)
 }
 
 // chy_solve_batch (steps = 0) and chy_solve_batch_refined (steps > 0)
+// One group of a batch (fields on identically described grids, one launch per
+// kernel), replayed from a captured HIP graph when the same group with the
+// same arrays and arguments ran before on the same stream (graphs on, stage
+// timing off); the capture holds kernel launches only.
+static int run_group(chy_ctx *ctx, const chy_solve_item &lead, const std::vector<chy_grid *> &grids,
+                     const Fields &F, hipStream_t s, int np, int steps)
+{
+    auto eager = [&]() {
+        if (steps > 0)
+            return run_refined_fields(lead.grid, grids, F, s, lead.maxtrm, lead.isym, lead.redge, np,
+                                      lead.coef_mode, steps, false);
+        return run_solve_fields(lead.grid, F, s, lead.maxtrm, lead.isym, lead.redge, np, lead.coef_mode);
+    };
+    if (ctx->prof.on || !ctx->graphs) return eager();
+    std::vector<uintptr_t> key(kGroupKeyHead + kGroupKeyPerField * (size_t)F.nf);
+    uint64_t rbits;
+    std::memcpy(&rbits, &lead.redge, sizeof(rbits));
+    key[0] = (uintptr_t)steps; key[1] = (uintptr_t)lead.maxtrm; key[2] = (uintptr_t)(lead.isym + 8);
+    key[3] = (uintptr_t)np; key[4] = (uintptr_t)lead.coef_mode; key[5] = (uintptr_t)rbits;
+    key[6] = (uintptr_t)s; key[7] = (uintptr_t)F.nf;
+    for (int f = 0; f < F.nf; ++f) {
+        uintptr_t *k = key.data() + kGroupKeyHead + kGroupKeyPerField * f;
+        k[0] = (uintptr_t)grids[f]; k[1] = (uintptr_t)F.rho[f]; k[2] = (uintptr_t)F.phi[f];
+        k[3] = (uintptr_t)F.coef[f]; k[4] = (uintptr_t)(uint32_t)F.klocat_in[f];
+    }
+    auto &cache = ctx->group_graphs;
+    size_t hit = cache.size();
+    for (size_t i = 0; i < cache.size(); ++i)
+        if (cache[i].key == key) hit = i;
+    if (hit == cache.size()) {
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return dev_rc((int)e, "hipStreamBeginCapture");
+        int rc = eager();
+        e = hipStreamEndCapture(s, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return dev_rc((int)e, "hipStreamEndCapture");
+        chy_ctx::GroupGraph g;
+        g.key = std::move(key);
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return dev_rc((int)e, "hipGraphInstantiate");
+        e = hipEventCreateWithFlags(&g.done, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            (void)hipGraphExecDestroy(g.exec);
+            return dev_rc((int)e, "hipEventCreate");
+        }
+        if (cache.size() >= kGroupGraphCache) {
+            retire_group_graph(cache.front());
+            cache.erase(cache.begin());
+        }
+        cache.push_back(std::move(g));
+        hit = cache.size() - 1;
+    } else {
+        std::rotate(cache.begin() + (long)hit, cache.begin() + (long)hit + 1, cache.end());
+        hit = cache.size() - 1;
+    }
+    int rc = dev_rc((int)hipGraphLaunch(cache[hit].exec, s), "hipGraphLaunch");
+    if (rc == CHY_OK) rc = dev_rc((int)hipEventRecord(cache[hit].done, s), "hipEventRecord");
+    return rc;
+}
+
 static int solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n, int steps)
 {
     if (!ctx || n < 0 || (n > 0 && !items)) return CHY_EINVAL;
@@ -714,11 +813,7 @@ static int solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n, int ste
                     set_field(F, f, it.grid, it.rho_d, it.phi_d, it.coef_d, it.diag ? it.diag->klocat : 0);
                     grids[f] = it.grid;
                 }
-                if (steps > 0)
-                    rc = run_refined_fields(lead.grid, grids, F, s, lead.maxtrm, lead.isym, lead.redge, np,
-                                            lead.coef_mode, steps, false);
-                else
-                    rc = run_solve_fields(lead.grid, F, s, lead.maxtrm, lead.isym, lead.redge, np, lead.coef_mode);
+                rc = run_group(ctx, lead, grids, F, s, np, steps);
             }
             for (size_t f = 0; f < grp.size() && rc == CHY_OK; ++f) {
                 const chy_solve_item &it = items[grp[f]];
@@ -829,8 +924,10 @@ int chy_grid_set_mode_solver(chy_grid *g, int kind)
     if (!g || (kind != CHY_MODES_CR && kind != CHY_MODES_EIGEN)) return CHY_EINVAL;
     if (kind == CHY_MODES_CR && !cr_supported(g)) return CHY_EUNSUPPORTED;
     if (kind == g->mode_solver) return CHY_OK;
+    DeviceScope ds(g->ctx->device);
     for (auto &e : g->graphs) retire_graph(e);
     g->graphs.clear();
+    drop_group_graphs(g->ctx, g);
     g->mode_solver = kind;
     return CHY_OK;
 }
@@ -843,11 +940,13 @@ int chy_grid_set_tune(chy_grid *g, int param, int value)
     int *slot = nullptr;
     if (param == CHY_TUNE_CR_LANES && cr_lanes_ok(g, value)) slot = &g->cr_lanes;
     if (param == CHY_TUNE_REFINE_M64 && value >= -1 && value <= g->N) slot = &g->refine_m64;
-    if (param == CHY_TUNE_REFINE_FUSED && (value == 0 || value == 1)) slot = &g->refine_fused;
+    if (param == CHY_TUNE_REFINE_FUSED && value >= 0 && value <= kMaxBatch) slot = &g->refine_fused;
     if (!slot) return CHY_EINVAL;
     if (*slot == value) return CHY_OK;
+    DeviceScope ds(g->ctx->device);
     for (auto &e : g->graphs) retire_graph(e);
     g->graphs.clear();
+    drop_group_graphs(g->ctx, g);
     *slot = value;
     return CHY_OK;
 }
@@ -868,6 +967,7 @@ int chy_grid_destroy(chy_grid *g)
     (void)hipStreamSynchronize(g->ctx->stream);
     for (auto &e : g->graphs) retire_graph(e);
     g->graphs.clear();
+    drop_group_graphs(g->ctx, g);
     free_grid_memory(g);
     delete g;
     return CHY_OK;

@@ -191,6 +191,14 @@ struct chy_ctx {
     int force_cap = 0;
     // chy_find_force_noalloc: the device's fixed scratch (force.hip)
     double *force_scratch = nullptr;
+    // chy_solve_batch: captured graphs of multi-field groups, keyed on the
+    // group's grids, arrays and arguments, most recently used last
+    struct GroupGraph {
+        std::vector<uintptr_t> key;
+        hipGraphExec_t exec = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    std::vector<GroupGraph> group_graphs;
 };
 
 namespace chy {
@@ -227,7 +235,7 @@ struct chy_grid {
     int mode_solver = CHY_MODES_CR;  // CHY_MODES_CR or CHY_MODES_EIGEN
     int cr_lanes = 64;               // lanes per r system in the cyclic reduction (64 or 32)
     int refine_m64 = 4;              // refinement: orders m <= this solved in binary64
-    int refine_fused = 0;            // refinement: one kernel per mode plane where the grid allows it
+    int refine_fused = 0;            // refinement: one kernel per mode plane for launches of up to this many fields
     size_t device_bytes = 0;
     std::vector<void *> allocations;
     chy::DeviceGrid d;

@@ -129,13 +129,14 @@ int chy_grid_mode_solver(const chy_grid *g);
  *   CHY_TUNE_REFINE_M64  refined solves: the r solves of orders m <= this
  *                        value run in binary64, the others in binary32
  *                        (-1..LMAX/2; default 4)
- *   CHY_TUNE_REFINE_FUSED refined solves of 1 or 2 steps: 1 runs all mode
- *                        work of a mode plane in one kernel with the binary32
- *                        plane in LDS where it fits (KMAX <= 129, JMAX <=
- *                        257): a third of the HBM traffic, but one block per
- *                        CU runs its phases one after another, and at
- *                        256x128x256 it is no faster; 0 (default) one kernel
- *                        per stage
+ *   CHY_TUNE_REFINE_FUSED refined solves of 1 or 2 steps launched for at
+ *                        most this many fields (0..32; default 0) run all
+ *                        mode work of a mode plane in one kernel with the
+ *                        binary32 plane in LDS, where it fits (KMAX <= 129,
+ *                        JMAX <= 257): a third of the HBM traffic, but one
+ *                        block per CU runs its phases one after another, so
+ *                        it wins for few fields per launch and loses for
+ *                        many; larger launches run one kernel per stage
  * Results do not depend on the shapes beyond rounding.  Changing a value
  * drops the grid's captured solve graphs.  CHY_EINVAL for a value the grid
  * cannot take. */

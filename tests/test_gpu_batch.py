@@ -239,3 +239,48 @@ def test_config4_64_heterogeneous_fields_refined(ctx):
         assert rel_max(chypot.to_host(jobs[i]["phi"]), want["phi"]) < 1e-10, g.name
     for j in jobs:
         j["grid"].close()
+
+
+@pytest.mark.parametrize("steps", [0, 2])
+def test_group_graph_replays_equal_eager(ctx, steps):
+    """Multi-field groups replay from captured HIP graphs: four calls of the
+    same batch with graphs on (capture, then three replays), with PHI cleared
+    before each and an eager batch into other buffers after each, must each
+    equal the eager batch bit for bit; a group that changes an array is
+    captured anew."""
+    g = fields.unit_grid(64, 32, 64)
+    pgs = [chypot.PreparedGrid(ctx, g) for _ in range(4)] + [chypot.PreparedGrid(ctx, fields.unit_grid(32, 16, 64))]
+    rhos = [chypot.to_device(fields.density_spiral_torus(pg.grid, seed=60 + i)) for i, pg in enumerate(pgs)]
+
+    def batch(phis):
+        return [dict(grid=pg, rho=r, phi=p, coef=torch.zeros(pg.coef_shape, dtype=torch.float64, device="cuda"),
+                     coef_mode=chypot.COEF_K2) for pg, r, p in zip(pgs, rhos, phis)]
+
+    ctx.graphs(False)
+    want = [torch.zeros_like(r) for r in rhos]
+    ctx.solve_batch(batch(want), refine_steps=steps)
+    ctx.sync()
+    ctx.graphs(True)
+    phis = [torch.zeros_like(r) for r in rhos]
+    jobs = batch(phis)
+    for rep in range(4):
+        for p in phis:
+            p.zero_()
+        ctx.solve_batch(jobs, refine_steps=steps)
+        ctx.sync()
+        for i, (p, w) in enumerate(zip(phis, want)):
+            assert torch.equal(p, w), (rep, i)
+        # eager work between the replays
+        ctx.graphs(False)
+        other = [torch.zeros_like(r) for r in rhos]
+        ctx.solve_batch(batch(other), refine_steps=steps)
+        ctx.sync()
+        ctx.graphs(True)
+    # another PHI array for one field: a new capture, still the eager result
+    phis2 = list(phis)
+    phis2[1] = torch.zeros_like(rhos[1])
+    ctx.solve_batch(batch(phis2), refine_steps=steps)
+    ctx.sync()
+    assert torch.equal(phis2[1], want[1])
+    for pg in pgs:
+        pg.close()

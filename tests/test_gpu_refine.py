@@ -154,7 +154,7 @@ def test_fused_planes_match_stage_kernels(ctx, jkl, steps):
     out = {}
     for fused in (True, False):
         pg.refine_fused = fused
-        assert pg.refine_fused == fused
+        assert pg.refine_fused == (32 if fused else 0)
         phi = torch.zeros_like(rho_d)
         pg.solve_refined(rho_d, phi, steps=steps)
         ctx.sync()
@@ -207,3 +207,30 @@ def test_refined_rejects_bad_steps(ctx):
             pg.solve_refined(rho, torch.zeros_like(rho), steps=steps)
         assert ei.value.code == chypot.CHY_EINVAL
     pg.close()
+
+
+def test_autotune_small_grid(ctx):
+    """The autotuner on a small grid: it picks candidates the grid accepts,
+    applies them, and the tuned refined solve still meets the 1e-10
+    criterion against the plain solve for one field and a batch."""
+    from chypot import autotune
+    g = fields.unit_grid(64, 32, 64)
+    table = autotune.tune(ctx, [g], batch_sizes=(1, 4), refine_steps=2, reps=2)
+    e = table["64x32x64"]
+    print(f"[gpu] autotune 64x32x64: {e}", flush=True)
+    assert e["cr_lanes"] in (64, 32) and 0 <= e["refine_fused"] <= 4
+    pgs = [chypot.PreparedGrid(ctx, g) for _ in range(4)]
+    for pg in pgs:
+        assert autotune.apply(pg, table)
+        assert pg.cr_lanes == e["cr_lanes"] and pg.refine_fused == e["refine_fused"]
+    rhos = [chypot.to_device(fields.density_spiral_torus(g, seed=40 + i)) for i in range(4)]
+    jobs = [dict(grid=pg, rho=r, phi=torch.zeros_like(r)) for pg, r in zip(pgs, rhos)]
+    ctx.solve_batch(jobs[:1], refine_steps=2)
+    ctx.solve_batch(jobs[1:], refine_steps=2)
+    for j in jobs:
+        plain = torch.zeros_like(j["rho"])
+        j["grid"].solve(j["rho"], plain)
+        ctx.sync()
+        assert float((j["phi"] - plain).abs().max()) / float(plain.abs().max()) < 1e-10
+    for pg in pgs:
+        pg.close()

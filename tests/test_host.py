@@ -120,3 +120,20 @@ def test_device_arguments_are_checked_against_grid_and_device():
     # COEF(JMAX+2, KMAX+2, LMAX/2, 2) in Fortran order is a (2, LMAX/2, KMAX+2, JMAX+2) tensor
     with pytest.raises(ValueError, match="shape"):
         chypot._dptr(_FakeCudaTensor((18, 10, 8, 2)), (2, 8, 10, 18), 0, "coef")
+
+
+def test_autotune_fused_threshold():
+    """The one-kernel-per-plane threshold is the largest batch size up to
+    which that form won at every timed size."""
+    from chypot import autotune
+    assert autotune.fused_threshold({1: 0.3, 8: 2.0, 32: 9.0}, {1: 0.4, 8: 2.5, 32: 8.0}) == 8
+    assert autotune.fused_threshold({1: 0.5, 32: 7.0}, {1: 0.4, 32: 8.0}) == 0
+    assert autotune.fused_threshold({1: 0.3, 32: 7.0}, {1: 0.4, 32: 8.0}) == 32
+    assert autotune.fused_threshold({}, {}) == 0
+
+
+def test_autotune_table_round_trip(tmp_path):
+    from chypot import autotune
+    t = {"256x128x256": {"cr_lanes": 64, "refine_fused": 1, "timings_ms": {"cr_lanes": {"64": 15.0}}}}
+    autotune.save(t, tmp_path / "t.json")
+    assert autotune.load(tmp_path / "t.json") == t
