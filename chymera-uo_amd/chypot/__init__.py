@@ -98,6 +98,13 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"libchypot.so not built: {LIB_PATH} (run __graft_entry__.build())")
+        try:
+            # torch first: the library then binds to the HIP runtime torch
+            # loaded, so both see the device (the other order leaves torch
+            # with a runtime that finds no GPU)
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         vp, dp = C.c_void_p, C.POINTER(C.c_double)
         L.chy_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]

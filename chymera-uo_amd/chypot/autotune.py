@@ -22,6 +22,8 @@ only), so tuning never changes what a caller gets beyond that.
 The result is a table {"JxKxL": {"cr_lanes": .., "refine_fused": ..,
 "timings_ms": {...}}} that apply() puts on prepared grids and save()/load()
 keep as JSON.  chypot.serve.Server applies a table to the grids it prepares.
+measured_table() is the table this tuner made on an MI355X for nine grid
+classes.
 """
 from __future__ import annotations
 
@@ -48,10 +50,10 @@ def fused_threshold(times_fused: dict, times_stages: dict) -> int:
     return best
 
 
-def _time_batch(ctx, jobs, steps, reps, warmup):
+def _time_batch(ctx, jobs, steps, reps, warmup, stream):
+    """Median milliseconds of a batched call, timed with events on the
+    stream the context issues on (the batch joins back to it)."""
     import torch
-    stream = torch.cuda.current_stream()
-    ctx.set_stream(stream)
     for _ in range(warmup):
         ctx.solve_batch(jobs, refine_steps=steps)
     t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -67,14 +69,18 @@ def _time_batch(ctx, jobs, steps, reps, warmup):
 
 def tune_grid(ctx: Context, g, batch_sizes=(1, 32), refine_steps: int = 2, reps: int = 5, warmup: int = 2):
     """Time the candidates on grid g (a chypot.fields.Grid); returns the
-    table entry for its class."""
+    table entry for its class.  The context issues on a stream of the
+    tuner's own meanwhile and on its own stream afterwards."""
     import torch
+    stream = torch.cuda.Stream()
+    ctx.set_stream(stream)
     nmax = max(batch_sizes)
     grids = [PreparedGrid(ctx, g) for _ in range(nmax)]
     jobs = []
     for i, pg in enumerate(grids):
         rho = fields.torus_device(g, 0.45, 0.25, 1.5, 0.05, 5000 + i)
         jobs.append(dict(grid=pg, rho=rho, phi=torch.zeros(pg.field_shape, dtype=torch.float64, device="cuda")))
+    torch.cuda.synchronize()  # the fields were made on torch's stream
     entry = {"cr_lanes": grids[0].cr_lanes, "refine_fused": grids[0].refine_fused, "timings_ms": {}}
     try:
         # lanes per r system of the cyclic reduction (plain solve)
@@ -86,7 +92,7 @@ def tune_grid(ctx: Context, g, batch_sizes=(1, 32), refine_steps: int = 2, reps:
                         pg.cr_lanes = lanes
                 except Exception:
                     continue  # not available on this grid
-                lane_ms[lanes] = _time_batch(ctx, jobs, 0, reps, warmup)
+                lane_ms[lanes] = _time_batch(ctx, jobs, 0, reps, warmup, stream)
             entry["timings_ms"]["cr_lanes"] = {str(k): v for k, v in lane_ms.items()}
             entry["cr_lanes"] = min(lane_ms, key=lane_ms.get)
             for pg in grids:
@@ -99,12 +105,14 @@ def tune_grid(ctx: Context, g, batch_sizes=(1, 32), refine_steps: int = 2, reps:
                 for fused, out in ((True, tf), (False, ts)):
                     for j in sub:
                         j["grid"].refine_fused = fused
-                    out[b] = _time_batch(ctx, sub, refine_steps, reps, warmup)
+                    out[b] = _time_batch(ctx, sub, refine_steps, reps, warmup, stream)
             entry["timings_ms"]["refine_fused"] = {str(b): {"one_kernel": tf[b], "per_stage": ts[b]} for b in tf}
             entry["refine_fused"] = fused_threshold(tf, ts)
     finally:
+        stream.synchronize()
         for pg in grids:
             pg.close()
+        ctx.set_stream(None)
     return entry
 
 
@@ -122,6 +130,14 @@ def apply(pg: PreparedGrid, table: dict) -> bool:
     pg.cr_lanes = int(e["cr_lanes"])
     pg.refine_fused = int(e["refine_fused"])
     return True
+
+
+def measured_table() -> dict:
+    """The table this tuner produced on an MI355X for nine grid classes
+    (tuned_mi355x.json beside this module): batch sizes 1, 8, 32 (1, 8 for
+    JMAX = 512), refined solves of 2 steps, medians of 3 calls."""
+    import pathlib
+    return load(pathlib.Path(__file__).with_name("tuned_mi355x.json"))
 
 
 def save(table: dict, path) -> None:

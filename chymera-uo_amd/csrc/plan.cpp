@@ -327,6 +327,9 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
     // ---- BLKTRI's root tables and the cyclic reduction's programs (crplan.cpp) ----
     CHY_TRY(build_cr_plan(g->cr, NZ, an.data(), bn.data(), cn.data()));
     g->mode_solver = cr_supported(g) ? CHY_MODES_CR : CHY_MODES_EIGEN;
+    // two orders per wavefront where the r system fits 32 lanes: measured
+    // faster or equal on every grid class tried (chypot/tuned_mi355x.json)
+    g->cr_lanes = cr_lanes_ok(g, 32) ? 32 : 64;
 
     // ---- FFT twiddles (exact angles; the reference uses recurrences, fft.f:225-232) ----
     std::vector<double2> twL(L), tw64(64);

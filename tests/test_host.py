@@ -137,3 +137,25 @@ def test_autotune_table_round_trip(tmp_path):
     t = {"256x128x256": {"cr_lanes": 64, "refine_fused": 1, "timings_ms": {"cr_lanes": {"64": 15.0}}}}
     autotune.save(t, tmp_path / "t.json")
     assert autotune.load(tmp_path / "t.json") == t
+
+
+def test_autotune_measured_table():
+    """The shipped MI355X table: every entry names candidates the grid class
+    takes (32 lanes only where JMAX-1 <= 256; the one-kernel refined form only
+    for KMAX <= 128) and its choice is the fastest timed candidate."""
+    from chypot import autotune
+    t = autotune.measured_table()
+    assert "256x128x256" in t
+    for key, e in t.items():
+        j, k, l = map(int, key.split("x"))
+        lanes = e["timings_ms"]["cr_lanes"]
+        assert e["cr_lanes"] == int(min(lanes, key=lanes.get))
+        if j - 1 > 256:
+            assert set(lanes) == {"64"}
+        if "refine_fused" in e["timings_ms"]:
+            assert k <= 128
+            tf = {int(b): v["one_kernel"] for b, v in e["timings_ms"]["refine_fused"].items()}
+            ts = {int(b): v["per_stage"] for b, v in e["timings_ms"]["refine_fused"].items()}
+            assert e["refine_fused"] == autotune.fused_threshold(tf, ts)
+        else:
+            assert k > 128 or e["refine_fused"] == 0
