@@ -396,6 +396,9 @@ def main():
                          "(single-field solves after the timed region; 0 skips)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the library's default launch shapes instead of timing the candidates first")
+    ap.add_argument("--cr-lanes", type=int, default=0, help="set the cyclic reduction's lanes per r system (64, 32)")
+    ap.add_argument("--refine-fused", type=int, default=-1,
+                    help="set the largest fields-per-launch of the one-kernel-per-plane refined solve")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the solves, copy a buffer of this many MiB device to device "
                          "three times (a kernel of known traffic for scaling HBM counters)")
@@ -440,6 +443,11 @@ def main():
             autotune.apply(j["grid"], tune_table)
         ctx.set_stream(stream)
         tune_table[autotune.grid_key(g)]["seconds"] = round(time.perf_counter() - t_tune, 2)
+    for j in jobs:  # shapes given on the command line win
+        if args.cr_lanes:
+            j["grid"].cr_lanes = args.cr_lanes
+        if args.refine_fused >= 0:
+            j["grid"].refine_fused = args.refine_fused
 
     if args.copy_mib > 0:
         src = torch.ones(args.copy_mib << 17, dtype=torch.float64, device="cuda")
