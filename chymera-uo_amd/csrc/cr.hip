@@ -281,7 +281,7 @@ __device__ __forceinline__ void zero_vec(double (&d)[R])
 //     in a fixed order ((0 + 1) + (2 + 3)), so results do not depend on timing.
 // LDS slots hold one wavefront's two vectors, element (r, rhs, lane) at
 // ((slot * R + r) * 2 + rhs) * 64 + lane (consecutive lanes, no bank clash).
-template <int R, int G, int W>
+template <int R, int G, int W, int KIND>
 __global__ __launch_bounds__(64 * W) void k_cr_group(CrArgs a, const Fields F, CrPhase ph)
 {
     constexpr int SPW = 64 / G;
@@ -344,14 +344,14 @@ __global__ __launch_bounds__(64 * W) void k_cr_group(CrArgs a, const Fields F, C
         return true;
     };
 
-    if (ph.kind == CR_PHASE_ODD) {
+    if constexpr (KIND == CR_PHASE_ODD) {
         // P(o) = pm(Y(o))  (pot/blktri.f:86-87, 106-107)
         load2(false, row.row, y0, y1);
         run_pf<R, G>(a, c, row.pm, gl, wave, W, y0, y1, s0, s1);
         if (reduce()) store2(true, row.row, s0, s1);
         return;
     }
-    if (ph.kind == CR_PHASE_EVEN) {
+    if constexpr (KIND == CR_PHASE_EVEN) {
         // Y(e) = (W1 + W2) - Y(e), W1 from P(e-h), W2 from P(e+h)  (pot/blktri.f:104-116)
         if (W == 1 || wave == 0) {
             load2(true, row.row - ph.h, s0, s1);
@@ -381,35 +381,37 @@ __global__ __launch_bounds__(64 * W) void k_cr_group(CrArgs a, const Fields F, C
         store2(false, row.row, y0, y1);
         return;
     }
-    // back substitution: Y(i) = pm((Y(i) - W1) - W2)  (pot/blktri.f:137-167)
-    double w10[R], w11[R], w20[R], w21[R];
-    zero_vec<R>(w10); zero_vec<R>(w11); zero_vec<R>(w20); zero_vec<R>(w21);
-    if (row.pa >= 0 && (W == 1 || wave == 0)) {
-        load2(false, row.row - ph.h, w10, w11);
-        run_chain<R, G>(a, c, row.pa, gl, w10, w11);
-    }
-    if (row.pc >= 0 && (W == 1 || wave == 1)) {
-        load2(false, row.row + ph.h, w20, w21);
-        run_chain<R, G>(a, c, row.pc, gl, w20, w21);
-    }
-    if constexpr (W > 1) {
-        if (wave == 0) put(W, w10, w11);
-        if (wave == 1) put(W + 1, w20, w21);
-        __syncthreads();
+    if constexpr (KIND == CR_PHASE_BACK) {
+        // back substitution: Y(i) = pm((Y(i) - W1) - W2)  (pot/blktri.f:137-167)
+        double w10[R], w11[R], w20[R], w21[R];
+        zero_vec<R>(w10); zero_vec<R>(w11); zero_vec<R>(w20); zero_vec<R>(w21);
+        if (row.pa >= 0 && (W == 1 || wave == 0)) {
+            load2(false, row.row - ph.h, w10, w11);
+            run_chain<R, G>(a, c, row.pa, gl, w10, w11);
+        }
+        if (row.pc >= 0 && (W == 1 || wave == 1)) {
+            load2(false, row.row + ph.h, w20, w21);
+            run_chain<R, G>(a, c, row.pc, gl, w20, w21);
+        }
+        if constexpr (W > 1) {
+            if (wave == 0) put(W, w10, w11);
+            if (wave == 1) put(W + 1, w20, w21);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                w10[r] = get(W, r, 0); w11[r] = get(W, r, 1);
+                w20[r] = get(W + 1, r, 0); w21[r] = get(W + 1, r, 1);
+            }
+        }
+        load2(false, row.row, y0, y1);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            w10[r] = get(W, r, 0); w11[r] = get(W, r, 1);
-            w20[r] = get(W + 1, r, 0); w21[r] = get(W + 1, r, 1);
+            y0[r] = (y0[r] - w10[r]) - w20[r];
+            y1[r] = (y1[r] - w11[r]) - w21[r];
         }
+        run_pf<R, G>(a, c, row.pm, gl, wave, W, y0, y1, s0, s1);
+        if (reduce()) store2(false, row.row, s0, s1);
     }
-    load2(false, row.row, y0, y1);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        y0[r] = (y0[r] - w10[r]) - w20[r];
-        y1[r] = (y1[r] - w11[r]) - w21[r];
-    }
-    run_pf<R, G>(a, c, row.pm, gl, wave, W, y0, y1, s0, s1);
-    if (reduce()) store2(false, row.row, s0, s1);
 }
 
 // Wavefronts per task for a phase: the partial-fraction products of level h
@@ -421,6 +423,18 @@ int phase_waves(const CrPhase &ph)
     return ph.kind == CR_PHASE_EVEN ? 2 : 4;
 }
 
+// one phase, its kernel specialised on the phase kind (each kind keeps only
+// its own vectors live)
+template <int R, int G, int KIND>
+void launch_phase(const CrPhase &ph, dim3 grid, const CrArgs &a, const Fields &F, hipStream_t s)
+{
+    switch (phase_waves(ph)) {
+    case 1: hipLaunchKernelGGL((k_cr_group<R, G, 1, KIND>), grid, dim3(64), 0, s, a, F, ph); break;
+    case 2: hipLaunchKernelGGL((k_cr_group<R, G, 2, KIND>), grid, dim3(128), 0, s, a, F, ph); break;
+    default: hipLaunchKernelGGL((k_cr_group<R, G, 4, KIND>), grid, dim3(256), 0, s, a, F, ph); break;
+    }
+}
+
 template <int R, int G>
 int launch_phases(const chy_grid *g, const CrArgs &a, const Fields &F, hipStream_t s)
 {
@@ -429,10 +443,10 @@ int launch_phases(const chy_grid *g, const CrArgs &a, const Fields &F, hipStream
     for (const CrPhase &ph : g->cr.phases) {
         if (ph.nrow <= 0) continue;
         const dim3 grid((unsigned)(ph.nrow * nw), F.nf);
-        switch (phase_waves(ph)) {
-        case 1: hipLaunchKernelGGL((k_cr_group<R, G, 1>), grid, dim3(64), 0, s, a, F, ph); break;
-        case 2: hipLaunchKernelGGL((k_cr_group<R, G, 2>), grid, dim3(128), 0, s, a, F, ph); break;
-        default: hipLaunchKernelGGL((k_cr_group<R, G, 4>), grid, dim3(256), 0, s, a, F, ph); break;
+        switch (ph.kind) {
+        case CR_PHASE_ODD: launch_phase<R, G, CR_PHASE_ODD>(ph, grid, a, F, s); break;
+        case CR_PHASE_EVEN: launch_phase<R, G, CR_PHASE_EVEN>(ph, grid, a, F, s); break;
+        default: launch_phase<R, G, CR_PHASE_BACK>(ph, grid, a, F, s); break;
         }
     }
     return (int)hipGetLastError();
