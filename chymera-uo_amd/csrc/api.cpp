@@ -396,7 +396,8 @@ static int ensure_refine(chy_grid *g)
     int rc = get((void **)&g->d.spec32, sizeof(float) * (size_t)g->L * g->K * g->J);
     if (!rc) rc = get((void **)&g->d.yh32, sizeof(float) * (size_t)g->L * g->PP * g->JP);
     if (!rc) rc = get((void **)&g->d.rstat, sizeof(unsigned long long) * 2 * kMaxRefine);
-    if (!rc && !g->d.fac32) {
+    // the r factors of the one-kernel-per-plane form, once a grid may use it
+    if (!rc && g->refine_fused > 0 && !g->d.fac32 && refine_planes_ok(g, 2)) {
         rc = get((void **)&g->d.fac32, sizeof(float) * 2 * (size_t)(g->N + 1) * g->PP * g->JP);
         if (!rc) rc = dev_rc(launch_thomas_factors(g, g->ctx->stream), "r-solve factor launch");
     }
@@ -462,7 +463,7 @@ static int run_refined_fields(chy_grid *g, const std::vector<chy_grid *> &grids,
     Profiler *p = g->ctx->prof.on ? &g->ctx->prof : nullptr;
     // the fused per-plane kernel where it applies (it collects no per-step
     // statistics: those calls take the kernel-per-stage path)
-    if (F.nf <= g->refine_fused && !with_stats && refine_planes_ok(g, steps)) {
+    if (F.nf <= g->refine_fused && g->d.fac32 && !with_stats && refine_planes_ok(g, steps)) {
         Fields Fp = F;
         for (int f = 0; f < F.nf; ++f) Fp.aux[f] = grids[f]->d.yh32;
         if (p) p->begin(CHY_STAGE_PLANES, s);

@@ -93,7 +93,8 @@ struct DeviceGrid {
     double *lam = nullptr;       // [PP]       eigenvalues (padded entries 1)
     float *wf32 = nullptr;       // [PP][PP]   wf rounded to binary32, zero past NZ (rows k, columns p)
     float *wi32 = nullptr;       // [PP][PP]   wi rounded to binary32, zero past NZ (rows p, columns k)
-    float *fac32 = nullptr;      // [N+1][JP][PP] x 2: LU factors of T_m + lambda_p (refined solves, first use)
+    float *fac32 = nullptr;      // [N+1][JP][PP] x 2: LU factors of T_m + lambda_p (the one-kernel-per-plane
+                                 // refined form; made by the first refined call after the form is enabled)
     // r operator (pot3.f:172-194,226-230): AM(1)=0, CM(M)=0, BM per azimuthal
     // order m; rows padded to JP (rows >= M: AM = CM = 0, BM = 1)
     double *am = nullptr;        // [JP]
