@@ -303,6 +303,9 @@ int launch_fft_inverse(const chy_grid *g, double *phi, hipStream_t s);
 int launch_fft_forward_f(const chy_grid *g, const Fields &F, int coef_mode, int lanes, bool moments,
                          bool spectrum, hipStream_t s);
 int launch_fft_inverse_f(const chy_grid *g, const Fields &F, hipStream_t s);
+// LMAX the transforms take: even, at most 1024, LMAX/2 with no prime factor
+// above 23 (powers of two by the specialised kernels, the rest mixed-radix)
+bool fft_size_ok(int L);
 
 // Mode solves on d.spec; rhs_edges applies the boundary terms of Y first
 // (skipped when the spectral BDYGEN has applied them).

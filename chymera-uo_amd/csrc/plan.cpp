@@ -217,7 +217,8 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
     if (J < 3 || K < 2 || L < 2) return CHY_EINVAL;
     // BLKTRI needs N = KMAX-1 = 2^k-1 with k >= 2 (pot/blktri.f:19-31)
     if (!is_pow2(K) || K < 4) return CHY_EUNSUPPORTED;
-    if (!is_pow2(L) || L > 1024) return CHY_EUNSUPPORTED;
+    // LMAX/2 may have prime factors up to 23, as in pot/fft.f:111-126
+    if (!fft_size_ok(L)) return CHY_EUNSUPPORTED;
     if (J > 1024) return CHY_EUNSUPPORTED;
     if (std::abs(isym) != 2) return CHY_EUNSUPPORTED;
     if (!(desc->dtheta > 0.0) || !std::isfinite(desc->pi) || !std::isfinite(desc->grav))

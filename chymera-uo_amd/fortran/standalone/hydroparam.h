@@ -2,7 +2,7 @@
 ! tree (tests and the potdiag driver).  The names and meanings are those the
 ! solver routines use from a CHYMERA build's headers/hydroparam.h; the sizes
 ! come from the preprocessor: -DCHY_JMAX=.. -DCHY_KMAX=.. -DCHY_LMAX=..
-! (KMAX a power of two, LMAX a power of two).
+! (KMAX a power of two, LMAX even with LMAX/2 23-smooth).
       integer jmax, jmax1, jmax2
       integer kmax, kmax1, kmax2
       integer lmax, lmax2

@@ -45,8 +45,10 @@ enum {
     CHY_OK = 0,
     CHY_EINVAL = 1,       /* bad argument or grid description */
     CHY_EUNSUPPORTED = 2, /* grid the solver cannot take: KMAX not 2^k (BLKTRI's own
-                             precondition, pot/blktri.f:29-31), LMAX not a power of
-                             two in [2,1024], |ISYM| != 2 */
+                             precondition, pot/blktri.f:29-31), LMAX odd or above
+                             1024, LMAX/2 with a prime factor above 23 (FFT's own
+                             limit, pot/fft.f:111-126), JMAX above 1024,
+                             |ISYM| != 2 */
     CHY_ENOMEM = 3,       /* device or host allocation failed */
     CHY_EDEVICE = 4,      /* HIP runtime error */
     CHY_ESTATE = 5,       /* call out of order (e.g. BDYGEN before SETBDY,

@@ -190,6 +190,15 @@ SOLVE_CASES = [
     ("g32_redge", fields.unit_grid(32, 16, 32), "small", (2, 10, 8, 0.9), True),
     ("g32x16x64_aniso", fields.Grid(32, 16, 64, 1.0 / 32, 0.75 / 16), "spiral", (2, 6, 9, 0.0), True),
     ("c1_64x32x64", fields.unit_grid(64, 32, 64), "small", (2, 10, 8, 0.0), False),
+    # LMAX/2 not a power of two: the reference's mixed-radix FFT (pot/fft.f:111-126)
+    ("g16x8x24_radix3", fields.unit_grid(16, 8, 24), "small", (2, 10, 8, 0.0), True),
+    ("g24x16x40_radix5", fields.unit_grid(24, 16, 40), "spiral", (2, 10, 8, 0.0), True),
+    ("g32x16x84_radix7", fields.unit_grid(32, 16, 84), "spiral", (-2, 10, 8, 0.0), False),
+    ("g16x8x286_radix11x13", fields.unit_grid(16, 8, 286), "small", (2, 10, 8, 0.0), False),
+    ("g16x8x68_radix17", fields.unit_grid(16, 8, 68), "spiral", (2, 8, 7, 0.0), True),
+    ("g16x8x76_radix19", fields.unit_grid(16, 8, 76), "small", (2, 10, 8, 0.0), True),
+    ("g16x8x46_radix23", fields.unit_grid(16, 8, 46), "spiral", (2, 10, 8, 0.0), True),
+    ("g32x16x120_radix3x5", fields.unit_grid(32, 16, 120), "spiral", (2, 10, 8, 0.0), False),
 ]
 
 

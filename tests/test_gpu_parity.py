@@ -9,7 +9,7 @@ import pytest
 
 import chypot
 from chypot import fields
-from helpers import GOLDEN, SMALL_CASES, load_case, phase_diff, rel_max
+from helpers import GOLDEN, MIXED_CASES, SMALL_CASES, assert_coef_close, load_case, phase_diff, rel_max
 
 pytestmark = pytest.mark.gpu
 
@@ -53,7 +53,7 @@ def run_solve(ctx, g, rho, params, coef_mode=chypot.COEF_FULL, phi_init=0.0, sol
 
 
 @pytest.mark.parametrize("solver", list(SOLVERS))
-@pytest.mark.parametrize("name", SMALL_CASES)
+@pytest.mark.parametrize("name", SMALL_CASES + MIXED_CASES)
 def test_solve_matches_reference(ctx, name, solver):
     g, rho, params, f = load_case(name)
     print(f"[gpu] solve {name} {g.name} {params} {solver}", flush=True)
@@ -69,17 +69,9 @@ def test_solve_matches_reference(ctx, name, solver):
     # ZAXPHI diagnostics: PHICHK is a difference of nearly equal numbers
     assert abs(res["phichk"] - float(f["phichk"])) < 1e-10
     assert res["klocat"] == int(f["klocat"])
-    ck = res["coef"][:, 1, :, :]
-    amp_ref, amp = f["coef_k2"][..., 0], ck[..., 0]
-    assert np.abs(amp - amp_ref).max() < 1e-11
-    sig = amp_ref > 1e-8
-    assert np.abs(phase_diff(ck[..., 1], f["coef_k2"][..., 1])[sig]).max() < 1e-6
+    assert_coef_close(res["coef"][:, 1, :, :], f["coef_k2"])
     if "coef" in f.files:
-        c = res["coef"]
-        rc = f["coef"]
-        assert np.abs(c[..., 0] - rc[..., 0]).max() < 1e-11
-        sig = rc[..., 0] > 1e-8
-        assert np.abs(phase_diff(c[..., 1], rc[..., 1])[sig]).max() < 1e-6
+        assert_coef_close(res["coef"], f["coef"])
 
 
 @pytest.mark.parametrize("npoint", range(1, 11))
@@ -273,18 +265,19 @@ def test_config2_eigenbasis_against_exact(ctx):
 
 
 @pytest.mark.parametrize("jkl,code", [((16, 6, 16), chypot.CHY_EUNSUPPORTED),
-                                      ((16, 8, 12), chypot.CHY_EUNSUPPORTED),
-                                      ((16, 8, 96), chypot.CHY_EUNSUPPORTED),
+                                      ((16, 8, 58), chypot.CHY_EUNSUPPORTED),
+                                      ((16, 8, 25), chypot.CHY_EUNSUPPORTED),
                                       ((16, 8, 2048), chypot.CHY_EUNSUPPORTED),
+                                      ((16, 8, 1026), chypot.CHY_EUNSUPPORTED),
                                       ((1040, 8, 16), chypot.CHY_EUNSUPPORTED),
                                       ((2, 8, 16), chypot.CHY_EINVAL)])
 def test_rejected_grids(ctx, jkl, code):
     """Grids the drop-in refuses with a code instead of solving wrongly:
-    KMAX not 2^k (BLKTRI's own precondition, pot/blktri.f:21-31); LMAX not a
-    power of two <= 1024, although the reference's FFT also takes LMAX/2 with
-    prime factors up to 23 (pot/fft.f:111-126; 96 = 2 x 48 is such a case):
-    the transforms here are radix-2 kernels specialised per size; JMAX above
-    1024, where the r solves would need more than 16 rows per lane; JMAX < 3."""
+    KMAX not 2^k (BLKTRI's own precondition, pot/blktri.f:21-31); LMAX/2
+    with a prime factor above 23 (58 = 2 x 29: the reference's FFT stops on
+    it too, pot/fft.f:111-126); odd LMAX (REALTR needs LMAX = 2N); LMAX above
+    1024 (the transforms keep a lane's spectrum in LDS); JMAX above 1024,
+    where the r solves would need more than 16 rows per lane; JMAX < 3."""
     g = fields.unit_grid(*jkl)
     with pytest.raises(chypot.ChyError) as ei:
         chypot.PreparedGrid(ctx, g)
@@ -350,15 +343,9 @@ def test_fused_solve_matches_reference(ctx, name, graphs, solver):
     assert abs(res["tmass"] - float(f["tmass_b"])) <= 1e-13 * abs(float(f["tmass_b"]))
     assert abs(res["phichk"] - float(f["phichk"])) < 1e-10
     assert res["klocat"] == int(f["klocat"])
-    c, rk = res["coef"], f["coef_k2"]
-    assert np.abs(c[:, 1, :, 0] - rk[..., 0]).max() < 1e-11
-    sig = rk[..., 0] > 1e-8
-    assert np.abs(phase_diff(c[:, 1, :, 1], rk[..., 1])[sig]).max() < 1e-6
+    assert_coef_close(res["coef"][:, 1], f["coef_k2"])
     if "coef" in f.files:
-        rc = f["coef"]
-        assert np.abs(c[..., 0] - rc[..., 0]).max() < 1e-11
-        sig = rc[..., 0] > 1e-8
-        assert np.abs(phase_diff(c[..., 1], rc[..., 1])[sig]).max() < 1e-6
+        assert_coef_close(res["coef"], f["coef"])
 
 
 def test_solve_graph_replays_match_eager(ctx):
@@ -433,14 +420,17 @@ def test_128x64x128_against_reference(ctx):
 
 
 @pytest.mark.parametrize("jkl", [(32, 256, 32), (512, 16, 32), (16, 8, 512), (40, 8, 16), (200, 32, 64),
-                                 (20, 4, 8), (64, 64, 2), (1024, 8, 16), (16, 8, 1024), (16, 512, 16)])
+                                 (20, 4, 8), (64, 64, 2), (1024, 8, 16), (16, 8, 1024), (16, 512, 16),
+                                 (128, 64, 192), (64, 32, 768), (16, 8, 1000), (16, 8, 6), (24, 8, 18)])
 def test_grid_shapes_against_oracle(ctx, jkl):
     """Shapes that select the other kernel variants: KMAX=256 (general z
     GEMM, eight reduction levels), JMAX=512 (eight r rows per lane), LMAX=512
     and LMAX<=16 transforms, JMAX not a multiple of 64, KMAX=4, LMAX=2
     (boundary values through PHI), and the largest grids the library takes in
     each direction: JMAX=1024 (sixteen r rows per lane: z eigenbasis only),
-    LMAX=1024, KMAX=512.
+    LMAX=1024, KMAX=512; mixed-radix transforms: LMAX/2 = 96 (one pass of
+    16, 2 and 3), 384 and 500 (4 lanes per block), 3 and 9 (no spectral
+    boundary lanes: LMAX/2 <= MAXTRM).
 
     Cyclic reduction (default where the grid allows it): against the oracle's
     POT3 by the same reduction, which is the reference's BLKTRI bit for bit;

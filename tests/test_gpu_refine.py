@@ -10,7 +10,7 @@ import pytest
 
 import chypot
 from chypot import fields
-from helpers import GOLDEN, SMALL_CASES, load_case, phase_diff, rel_max
+from helpers import GOLDEN, MIXED_CASES, SMALL_CASES, assert_coef_close, load_case, rel_max
 
 pytestmark = pytest.mark.gpu
 
@@ -34,7 +34,7 @@ def refined(ctx, pg, g, rho_d, params, steps, phi_init=0.0):
     return chypot.to_host(phi), np.asfortranarray(coef.cpu().numpy().transpose(3, 2, 1, 0)), d, rel
 
 
-@pytest.mark.parametrize("name", SMALL_CASES)
+@pytest.mark.parametrize("name", SMALL_CASES + MIXED_CASES)
 def test_refined_matches_reference(ctx, name):
     g, rho, params, f = load_case(name)
     pg = chypot.PreparedGrid(ctx, g, isym=params["isym"])
@@ -49,13 +49,10 @@ def test_refined_matches_reference(ctx, name):
     assert rel[1] < 1e-5 and rel[2] < 1e-9
     assert abs(d.tmass - float(f["tmass_b"])) <= 1e-13 * abs(float(f["tmass_b"]))
     assert abs(d.phichk - float(f["phichk"])) < 1e-8
-    ck, rk = coef[:, 1], f["coef_k2"]
-    assert np.abs(ck[..., 0] - rk[..., 0]).max() < 1e-11
-    sig = rk[..., 0] > 1e-8
-    assert np.abs(phase_diff(ck[..., 1], rk[..., 1])[sig]).max() < 1e-6
+    assert_coef_close(coef[:, 1], f["coef_k2"])
 
 
-@pytest.mark.parametrize("name", SMALL_CASES)
+@pytest.mark.parametrize("name", SMALL_CASES + MIXED_CASES)
 def test_refined_two_steps_matches_reference(ctx, name):
     """Two steps by the one-kernel-per-plane form against the reference's
     whole solve; the interior of PHI starts as NaN."""
