@@ -52,6 +52,14 @@ __device__ __forceinline__ double rcp_nr(double x)
     return __builtin_fma(r, e, r);
 }
 
+// r, or the identity row where !keep, by selects: a branch here makes the
+// wavefront wait for every shuffle in flight before it issues the next ones
+__device__ __forceinline__ Red identity_unless(bool keep, const Red &r)
+{
+    return Red{keep ? r.al : 0.0, keep ? r.be : 1.0, keep ? r.ga : 0.0,
+               keep ? r.d0 : 0.0, keep ? r.d1 : 0.0, keep ? r.inv : 1.0};
+}
+
 // neighbour s lanes below / above within a group of G lanes; identity rows
 // outside the group
 template <int G>
@@ -64,8 +72,7 @@ __device__ __forceinline__ Red red_up(const Red &r, int s, int gl)
     o.d0 = __shfl_up(r.d0, s, G);
     o.d1 = __shfl_up(r.d1, s, G);
     o.inv = __shfl_up(r.inv, s, G);
-    if (gl < s) o = Red{0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
-    return o;
+    return identity_unless(gl >= s, o);
 }
 
 template <int G>
@@ -78,8 +85,7 @@ __device__ __forceinline__ Red red_down(const Red &r, int s, int gl)
     o.d0 = __shfl_down(r.d0, s, G);
     o.d1 = __shfl_down(r.d1, s, G);
     o.inv = __shfl_down(r.inv, s, G);
-    if (gl + s >= G) o = Red{0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
-    return o;
+    return identity_unless(gl + s < G, o);
 }
 
 // (T_m - rt I) x = d for two right-hand sides over a group of G lanes (R*G

@@ -361,6 +361,15 @@ struct Red {
     AT d0, d1;      // right-hand sides (cosine, sine)
 };
 
+// r, or the identity row where !keep, by selects: a branch here makes the
+// wavefront wait for every shuffle in flight before it issues the next ones
+template <typename AT>
+__device__ __forceinline__ Red<AT> identity_unless(bool keep, const Red<AT> &r)
+{
+    return Red<AT>{keep ? r.al : (AT)0, keep ? r.be : (AT)1, keep ? r.ga : (AT)0, keep ? r.d0 : (AT)0,
+                   keep ? r.d1 : (AT)0};
+}
+
 template <typename AT>
 __device__ __forceinline__ Red<AT> shfl_up_red(const Red<AT> &r, int s, int lane)
 {
@@ -370,8 +379,7 @@ __device__ __forceinline__ Red<AT> shfl_up_red(const Red<AT> &r, int s, int lane
     o.ga = __shfl_up(r.ga, s, 64);
     o.d0 = __shfl_up(r.d0, s, 64);
     o.d1 = __shfl_up(r.d1, s, 64);
-    if (lane < s) o = Red<AT>{0, 1, 0, 0, 0};
-    return o;
+    return identity_unless(lane >= s, o);
 }
 
 template <typename AT>
@@ -383,8 +391,7 @@ __device__ __forceinline__ Red<AT> shfl_down_red(const Red<AT> &r, int s, int la
     o.ga = __shfl_down(r.ga, s, 64);
     o.d0 = __shfl_down(r.d0, s, 64);
     o.d1 = __shfl_down(r.d1, s, 64);
-    if (lane + s >= 64) o = Red<AT>{0, 1, 0, 0, 0};
-    return o;
+    return identity_unless(lane + s < 64, o);
 }
 
 // R consecutive elements at p (16-byte aligned) into registers of type AT,
@@ -841,8 +848,10 @@ __device__ __forceinline__ void rsolve1(const AT (&ca)[R], const AT (&cb)[R], co
         AT ld = __shfl_up(dd, s, 64);
         AT hal = __shfl_down(al, s, 64), hbe = __shfl_down(be, s, 64), hga = __shfl_down(ga, s, 64);
         AT hd = __shfl_down(dd, s, 64);
-        if (lane < s) { lal = (AT)0; lbe = (AT)1; lga = (AT)0; ld = (AT)0; }
-        if (lane + s >= 64) { hal = (AT)0; hbe = (AT)1; hga = (AT)0; hd = (AT)0; }
+        // identity rows past the ends, by selects (see identity_unless)
+        const bool kl = lane >= s, kh = lane + s < 64;
+        lal = kl ? lal : (AT)0; lbe = kl ? lbe : (AT)1; lga = kl ? lga : (AT)0; ld = kl ? ld : (AT)0;
+        hal = kh ? hal : (AT)0; hbe = kh ? hbe : (AT)1; hga = kh ? hga : (AT)0; hd = kh ? hd : (AT)0;
         const AT k1 = qdiv(al, lbe), k2 = qdiv(ga, hbe);
         const AT nal = -lal * k1, nga = -hga * k2;
         const AT nbe = be - lga * k1 - hal * k2;
