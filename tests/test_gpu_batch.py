@@ -284,3 +284,35 @@ def test_group_graph_replays_equal_eager(ctx, steps):
     assert torch.equal(phis2[1], want[1])
     for pg in pgs:
         pg.close()
+
+
+@pytest.mark.parametrize("steps", [0, 2])
+def test_batched_mixed_radix_fields_equal_single_solves(ctx, steps):
+    """Fields whose LMAX/2 is not a power of two (the mixed-radix transforms)
+    in batched launches beside a power-of-two shape: five fields at LMAX 40
+    (20 = 4 x 5) and three at LMAX 48 (24 = 8 x 3) on their own grid objects,
+    two at LMAX 64, in one batch, plain and refined; each equal bit for bit to
+    the same field solved alone."""
+    print(f"[gpu] batched mixed-radix fields, refine steps {steps}", flush=True)
+    jobs = [_own_grid_job(ctx, 700 + i, (24, 16, 40)) for i in range(5)]
+    jobs += [_own_grid_job(ctx, 710 + i, (32, 16, 48)) for i in range(3)]
+    jobs += [_own_grid_job(ctx, 720 + i, (32, 16, 64)) for i in range(2)]
+    try:
+        alone = []
+        for jb in jobs:
+            phi = torch.zeros_like(jb["phi"])
+            coef = torch.zeros_like(jb["coef"])
+            if steps:
+                jb["grid"].solve_refined(jb["rho"], phi, steps=steps, coef=coef, coef_mode=jb["coef_mode"])
+            else:
+                jb["grid"].solve(jb["rho"], phi, coef=coef, coef_mode=jb["coef_mode"])
+            ctx.sync()
+            alone.append((phi.cpu(), coef.cpu()))
+        ctx.solve_batch(jobs, refine_steps=steps)
+        ctx.sync()
+        for i, (jb, (phi, coef)) in enumerate(zip(jobs, alone)):
+            assert torch.equal(jb["phi"].cpu(), phi), f"PHI of item {i}"
+            assert torch.equal(jb["coef"].cpu(), coef), f"COEF of item {i}"
+    finally:
+        for jb in jobs:
+            jb["grid"].close()
