@@ -136,12 +136,18 @@ void Profiler::release()
     pool.clear();
 }
 
+// A failed record leaves the stage untimed (and clears the failure, which
+// would otherwise surface at the next launch check).
 void Profiler::begin(int stage, hipStream_t s)
 {
     if (!on) return;
     hipEvent_t e = take();
     if (!e) return;
-    (void)hipEventRecord(e, s);
+    if (hipEventRecord(e, s) != hipSuccess) {
+        (void)hipGetLastError();
+        pool.push_back(e);
+        return;
+    }
     open_stage = stage;
     open_ev = e;
 }
@@ -151,8 +157,13 @@ void Profiler::end(hipStream_t s)
     if (!on || open_stage < 0) return;
     hipEvent_t e = take();
     if (!e) return;
-    (void)hipEventRecord(e, s);
-    pending.push_back(Rec{open_stage, open_ev, e});
+    if (hipEventRecord(e, s) != hipSuccess) {
+        (void)hipGetLastError();
+        pool.push_back(e);
+        pool.push_back(open_ev);
+    } else {
+        pending.push_back(Rec{open_stage, open_ev, e});
+    }
     open_stage = -1;
     open_ev = nullptr;
 }
@@ -188,13 +199,26 @@ static bool bdygen_args_ok(int maxtrm, int isym, double redge)
     return maxtrm >= 0 && maxtrm <= kMaxTrm && std::abs(isym) == 2 && redge >= 0.0;
 }
 
+// Waits for a graph's last launch (its done event), then destroys the event.
+// Should the wait fail, the whole device is waited for instead, and the
+// failure is cleared: HIP keeps a failed call's error for the next
+// hipGetLastError on the thread, where a later launch check (ours or the
+// caller's) would take it for its own.
+static void wait_done(hipEvent_t done)
+{
+    if (!done) return;
+    if (hipEventSynchronize(done) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipDeviceSynchronize();
+        (void)hipGetLastError();
+    }
+    (void)hipEventDestroy(done);
+}
+
 // waits for the entry's last launch, then frees it
 static void retire_graph(chy_grid::GraphEntry &e)
 {
-    if (e.done) {
-        (void)hipEventSynchronize(e.done);
-        (void)hipEventDestroy(e.done);
-    }
+    wait_done(e.done);
     if (e.exec) (void)hipGraphExecDestroy(e.exec);
     e.done = nullptr;
     e.exec = nullptr;
@@ -202,10 +226,7 @@ static void retire_graph(chy_grid::GraphEntry &e)
 
 static void retire_group_graph(chy_ctx::GroupGraph &e)
 {
-    if (e.done) {
-        (void)hipEventSynchronize(e.done);
-        (void)hipEventDestroy(e.done);
-    }
+    wait_done(e.done);
     if (e.exec) (void)hipGraphExecDestroy(e.exec);
     e.done = nullptr;
     e.exec = nullptr;
@@ -629,16 +650,12 @@ int chy_ctx_destroy(chy_ctx *ctx)
     return CHY_OK;
 }
 
+// The worker streams already made stay until the context is destroyed (a
+// smaller count leaves the rest idle): the captured graphs' done events and
+// the stage timer's events refer to the streams they were recorded on.
 int chy_ctx_workers(chy_ctx *ctx, int n)
 {
     if (!ctx || n < 1 || n > CHY_MAX_WORKERS) return CHY_EINVAL;
-    DeviceScope ds(ctx->device);
-    if (n == ctx->nworkers) return CHY_OK;
-    int rc = dev_rc((int)hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    for (hipStream_t w : ctx->workers)
-        if (rc == CHY_OK) rc = dev_rc((int)hipStreamSynchronize(w), "hipStreamSynchronize");
-    if (rc) return rc;
-    release_workers(ctx);
     ctx->nworkers = n;
     return CHY_OK;
 }

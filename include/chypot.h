@@ -202,8 +202,10 @@ typedef struct {
 } chy_solve_item;
 #define CHY_MAX_WORKERS 16
 int chy_solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n);
-/* Number of worker streams for chy_solve_batch (1..CHY_MAX_WORKERS, default 4;
- * waits for the context's streams). */
+/* Number of worker streams chy_solve_batch uses (1..CHY_MAX_WORKERS, default
+ * 4).  Streams are made on first use and kept until chy_ctx_destroy; a
+ * smaller count leaves the others idle.  Work already issued is not waited
+ * for. */
 int chy_ctx_workers(chy_ctx *ctx, int n);
 
 /* Host-pointer forms used by the Fortran shims: rho_h/phi_h/coef_h are the

@@ -316,3 +316,43 @@ def test_batched_mixed_radix_fields_equal_single_solves(ctx, steps):
     finally:
         for jb in jobs:
             jb["grid"].close()
+
+
+def test_worker_counts_graph_caches_and_teardown(ctx):
+    """Changing the number of worker streams between batches, with graphs of
+    single fields and of multi-field groups cached on the streams used before,
+    a batch with stage timing, then closing every grid and the context: every
+    batch equals the eager one, and the caller's next GPU call finds no error
+    left behind (a failed wait in the cache clean-up must not surface there)."""
+    print("[gpu] worker counts, graph caches and teardown", flush=True)
+    c = chypot.Context(0)
+    jobs = [_own_grid_job(c, 800 + i, (32, 16, 64)) for i in range(3)]
+    jobs += [_own_grid_job(c, 810, (16, 32, 32)), _own_grid_job(c, 811, (24, 16, 40))]
+    try:
+        for steps in (2, 0):
+            c.graphs(False)
+            c.solve_batch(jobs, refine_steps=steps)
+            c.sync()
+            want = [jb["phi"].clone() for jb in jobs]
+            c.graphs(True)
+            for w in (8, 4, 16, 8, 1, 8):
+                c.workers(w)
+                for rep in range(2):
+                    for jb in jobs:
+                        jb["phi"].zero_()
+                    c.solve_batch(jobs, refine_steps=steps)
+                    c.sync()
+                    for i, (jb, wv) in enumerate(zip(jobs, want)):
+                        assert torch.equal(jb["phi"], wv), (steps, w, rep, i)
+            c.profile(True)
+            c.solve_batch(jobs, refine_steps=steps)
+            c.sync()
+            assert c.stage_times()["fft_fwd"][1] > 0
+            c.profile(False)
+    finally:
+        for jb in jobs:
+            jb["grid"].close()
+        c.close()
+    x = torch.ones(4096, dtype=torch.float64, device="cuda")
+    assert float(x.sum()) == 4096.0
+    torch.cuda.synchronize()
