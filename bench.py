@@ -194,7 +194,11 @@ def hetero_batch(ctx, n, refine_steps, reps=3):
                          coef=torch.zeros(pg.coef_shape, dtype=torch.float64, device="cuda"),
                          coef_mode=chypot.COEF_K2))
         points += g.jmax * g.kmax * g.lmax
-    out = {"fields": n, "distinct_grids": len(shapes), "refine_steps": refine_steps}
+    # many small groups: more worker streams overlap more of them (16 measured
+    # 5-6% faster than 8 on this set)
+    workers = 16
+    ctx.workers(workers)
+    out = {"fields": n, "distinct_grids": len(shapes), "refine_steps": refine_steps, "workers": workers}
     for label, steps in (("refined", refine_steps), ("plain_fp64", 0)):
         ctx.solve_batch(jobs, refine_steps=steps)
         torch.cuda.synchronize()
@@ -567,6 +571,7 @@ def main():
         out_paths["refined_one_kernel_per_plane"] = fused
 
     hetero = hetero_batch(ctx, args.hetero, args.refine_steps) if args.hetero > 0 else None
+    ctx.workers(W)
     serving = serve_run(ctx, args, tune_table) if args.serve > 0 else None
     ctx.set_stream(stream)
     source = (source_row(ctx, stream, g, args.source_calls, not args.no_cpu_baseline)

@@ -647,6 +647,7 @@ int chy_ctx_destroy(chy_ctx *ctx)
     if (ctx->force_h) (void)hipHostFree(ctx->force_h);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
+    (void)hipGetLastError();  // a failed release must not surface at the caller's next launch check
     return CHY_OK;
 }
 
@@ -982,12 +983,16 @@ int chy_grid_destroy(chy_grid *g)
 {
     if (!g) return CHY_EINVAL;
     DeviceScope ds(g->ctx->device);
-    (void)hipStreamSynchronize(g->ctx->stream);
+    if (hipStreamSynchronize(g->ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipDeviceSynchronize();
+    }
     for (auto &e : g->graphs) retire_graph(e);
     g->graphs.clear();
     drop_group_graphs(g->ctx, g);
     free_grid_memory(g);
     delete g;
+    (void)hipGetLastError();  // a failed release must not surface at the caller's next launch check
     return CHY_OK;
 }
 
