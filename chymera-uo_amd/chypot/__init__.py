@@ -177,8 +177,15 @@ class Context:
         self.device = device
 
     def set_stream(self, stream=None):
-        """Issue on a torch.cuda.Stream (or its raw handle); None = own stream."""
-        raw = getattr(stream, "cuda_stream", stream)
+        """Issue on a torch.cuda.Stream (or its raw handle); None = own stream.
+        The null (legacy default) stream is refused rather than replaced by
+        the context's own stream, which would not be ordered with it."""
+        raw = None
+        if stream is not None:
+            raw = getattr(stream, "cuda_stream", stream)
+            if not raw:
+                raise ValueError("set_stream: the null stream cannot be used; pass a created stream "
+                                 "(torch.cuda.Stream()) or None for the context's own")
         _check(lib().chy_ctx_set_stream(self.handle, C.c_void_p(raw) if raw else None),
                "chy_ctx_set_stream")
 

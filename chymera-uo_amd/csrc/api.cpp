@@ -26,6 +26,10 @@ void set_device_error(hipError_t e, const char *where)
 // grid, density, PHI, COEF and KLOCAT
 constexpr size_t kGroupKeyHead = 8, kGroupKeyPerField = 5;
 constexpr size_t kGroupGraphCache = 16;
+// a captured batch's key: kBatchKeyHead words, then per item its grid,
+// arrays, arguments, KLOCAT in and whether it wants diagnostics
+constexpr size_t kBatchKeyHead = 6, kBatchKeyPerItem = 11;
+constexpr size_t kBatchGraphCache = 4;
 
 static int dev_rc(int hip_rc, const char *where)
 {
@@ -232,14 +236,13 @@ static void retire_group_graph(chy_ctx::GroupGraph &e)
     e.exec = nullptr;
 }
 
-// drops the captured group graphs that launch work on grid g (all with g null)
-static void drop_group_graphs(chy_ctx *c, const chy_grid *g)
+// drops the cached graphs of v whose keys (head words, then per item a
+// record whose first word is the grid) name grid g (all with g null)
+static void drop_graphs_of(std::vector<chy_ctx::GroupGraph> &v, size_t head, size_t per, const chy_grid *g)
 {
-    auto &v = c->group_graphs;
     for (size_t i = 0; i < v.size();) {
         bool hit = g == nullptr;
-        for (size_t k = kGroupKeyHead; !hit && k < v[i].key.size(); k += kGroupKeyPerField)
-            hit = v[i].key[k] == (uintptr_t)g;
+        for (size_t k = head; !hit && k < v[i].key.size(); k += per) hit = v[i].key[k] == (uintptr_t)g;
         if (hit) {
             retire_group_graph(v[i]);
             v.erase(v.begin() + (long)i);
@@ -247,6 +250,26 @@ static void drop_group_graphs(chy_ctx *c, const chy_grid *g)
             ++i;
         }
     }
+}
+
+// drops the captured group and batch graphs that launch work on grid g (all
+// with g null)
+static void drop_group_graphs(chy_ctx *c, const chy_grid *g)
+{
+    drop_graphs_of(c->group_graphs, kGroupKeyHead, kGroupKeyPerField, g);
+    drop_graphs_of(c->batch_graphs, kBatchKeyHead, kBatchKeyPerItem, g);
+}
+
+// true while s records into a graph (a capture of ours or of the caller's):
+// work issued then must be plain launches, not graph replays or captures
+static bool capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return st != hipStreamCaptureStatusNone;
 }
 
 // BDYGEN alone: ring moments from a transform of the interior lanes (no
@@ -320,7 +343,7 @@ static int run_solve(chy_grid *g, hipStream_t s, int maxtrm, int isym, double re
 {
     if (!bdygen_args_ok(maxtrm, isym, redge)) return CHY_EINVAL;
     chy_ctx *c = g->ctx;
-    if (c->prof.on || !c->graphs)
+    if (c->prof.on || !c->graphs || capturing(s))
         return run_solve_eager(g, s, maxtrm, isym, redge, npoint, rho, phi, coef, coef_mode, klocat_in);
     SolveKey key{rho, phi, coef, coef_mode, maxtrm, isym, npoint, klocat_in, redge, s};
     hipGraphExec_t exec = nullptr;
@@ -675,7 +698,7 @@ static int run_group(chy_ctx *ctx, const chy_solve_item &lead, const std::vector
                                       lead.coef_mode, steps, false);
         return run_solve_fields(lead.grid, F, s, lead.maxtrm, lead.isym, lead.redge, np, lead.coef_mode);
     };
-    if (ctx->prof.on || !ctx->graphs) return eager();
+    if (ctx->prof.on || !ctx->graphs || capturing(s)) return eager();
     std::vector<uintptr_t> key(kGroupKeyHead + kGroupKeyPerField * (size_t)F.nf);
     uint64_t rbits;
     std::memcpy(&rbits, &lead.redge, sizeof(rbits));
@@ -727,37 +750,11 @@ static int run_group(chy_ctx *ctx, const chy_solve_item &lead, const std::vector
     return rc;
 }
 
-static int solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n, int steps)
+// The fork from the context's stream, the groups on the worker streams and
+// the join of one batch (solve_batch).
+static int issue_batch(chy_ctx *ctx, const chy_solve_item *items, int n, int steps)
 {
-    if (!ctx || n < 0 || (n > 0 && !items)) return CHY_EINVAL;
-    if (steps < 0 || steps > kMaxRefine) return CHY_EINVAL;
-    if (n == 0) return CHY_OK;
-    // validate everything before anything is launched
-    bool any_diag = false;
-    for (int i = 0; i < n; ++i) {
-        const chy_solve_item &it = items[i];
-        if (!it.grid) return CHY_ESTATE;
-        if (it.grid->ctx != ctx || !it.rho_d || !it.phi_d) return CHY_EINVAL;
-        int np = it.npoint;
-        int rc = check_pot3_args(it.grid, np, it.coef_mode, it.coef_d);
-        if (rc) return rc;
-        if (!bdygen_args_ok(it.maxtrm, it.isym, it.redge)) return CHY_EINVAL;
-        any_diag = any_diag || it.diag;
-    }
-    DeviceScope ds(ctx->device);
-    int rc = ensure_workers(ctx);
-    if (rc) return rc;
-    for (int i = 0; i < n && steps > 0 && rc == CHY_OK; ++i) rc = ensure_refine(items[i].grid);
-    if (rc) return rc;
-    if (any_diag && ctx->diag_cap < n) {
-        if (ctx->d_diag) (void)hipFree(ctx->d_diag);
-        if (ctx->h_diag) (void)hipHostFree(ctx->h_diag);
-        ctx->d_diag = ctx->h_diag = nullptr;
-        ctx->diag_cap = 0;
-        if (hipMalloc(&ctx->d_diag, sizeof(double) * 3 * n) != hipSuccess) return CHY_ENOMEM;
-        if (hipHostMalloc(&ctx->h_diag, sizeof(double) * 3 * n) != hipSuccess) return CHY_ENOMEM;
-        ctx->diag_cap = n;
-    }
+    int rc = CHY_OK;
     // Rounds: item i is in round r when r earlier items use its grid object
     // (they share its workspace).  Within a round, items on grids with the
     // same tables and the same BDYGEN/POT3 parameters form groups of up to
@@ -853,6 +850,106 @@ static int solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n, int ste
                                       "hipStreamWaitEvent");
         if (rc == CHY_OK) rc = r2;
     }
+    return rc;
+}
+
+// A batch replayed from a HIP graph of the whole fork, groups and join when
+// the same batch (grids, arrays, arguments, worker count, stream, diagnostics
+// staging) ran before; captured on first use.  Work on different grids stays
+// on different branches of the graph, as on the worker streams.
+static int run_batch_graph(chy_ctx *ctx, const chy_solve_item *items, int n, int steps, bool any_diag)
+{
+    std::vector<uintptr_t> key(kBatchKeyHead + kBatchKeyPerItem * (size_t)n);
+    key[0] = (uintptr_t)steps; key[1] = (uintptr_t)n; key[2] = (uintptr_t)ctx->nworkers;
+    key[3] = (uintptr_t)ctx->stream; key[4] = (uintptr_t)any_diag; key[5] = (uintptr_t)ctx->d_diag;
+    for (int i = 0; i < n; ++i) {
+        const chy_solve_item &it = items[i];
+        uint64_t rbits;
+        std::memcpy(&rbits, &it.redge, sizeof(rbits));
+        uintptr_t *k = key.data() + kBatchKeyHead + kBatchKeyPerItem * (size_t)i;
+        k[0] = (uintptr_t)it.grid; k[1] = (uintptr_t)it.rho_d; k[2] = (uintptr_t)it.phi_d;
+        k[3] = (uintptr_t)it.coef_d; k[4] = (uintptr_t)it.coef_mode; k[5] = (uintptr_t)it.maxtrm;
+        k[6] = (uintptr_t)(it.isym + 8); k[7] = (uintptr_t)it.npoint; k[8] = (uintptr_t)rbits;
+        k[9] = (uintptr_t)(uint32_t)(it.diag ? it.diag->klocat : 0); k[10] = (uintptr_t)(it.diag != nullptr);
+    }
+    auto &cache = ctx->batch_graphs;
+    size_t hit = cache.size();
+    for (size_t i = 0; i < cache.size(); ++i)
+        if (cache[i].key == key) hit = i;
+    if (hit == cache.size()) {
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return dev_rc((int)e, "hipStreamBeginCapture");
+        int rc = issue_batch(ctx, items, n, steps);
+        e = hipStreamEndCapture(ctx->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return dev_rc((int)e, "hipStreamEndCapture");
+        chy_ctx::GroupGraph b;
+        b.key = std::move(key);
+        e = hipGraphInstantiate(&b.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return dev_rc((int)e, "hipGraphInstantiate");
+        e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            (void)hipGraphExecDestroy(b.exec);
+            return dev_rc((int)e, "hipEventCreate");
+        }
+        if (cache.size() >= kBatchGraphCache) {
+            retire_group_graph(cache.front());
+            cache.erase(cache.begin());
+        }
+        cache.push_back(std::move(b));
+        hit = cache.size() - 1;
+    } else {
+        std::rotate(cache.begin() + (long)hit, cache.begin() + (long)hit + 1, cache.end());
+        hit = cache.size() - 1;
+    }
+    int rc = dev_rc((int)hipGraphLaunch(cache[hit].exec, ctx->stream), "hipGraphLaunch");
+    if (rc == CHY_OK) rc = dev_rc((int)hipEventRecord(cache[hit].done, ctx->stream), "hipEventRecord");
+    return rc;
+}
+
+static int solve_batch(chy_ctx *ctx, const chy_solve_item *items, int n, int steps)
+{
+    if (!ctx || n < 0 || (n > 0 && !items)) return CHY_EINVAL;
+    if (steps < 0 || steps > kMaxRefine) return CHY_EINVAL;
+    if (n == 0) return CHY_OK;
+    // validate everything before anything is launched
+    bool any_diag = false;
+    for (int i = 0; i < n; ++i) {
+        const chy_solve_item &it = items[i];
+        if (!it.grid) return CHY_ESTATE;
+        if (it.grid->ctx != ctx || !it.rho_d || !it.phi_d) return CHY_EINVAL;
+        int np = it.npoint;
+        int rc = check_pot3_args(it.grid, np, it.coef_mode, it.coef_d);
+        if (rc) return rc;
+        if (!bdygen_args_ok(it.maxtrm, it.isym, it.redge)) return CHY_EINVAL;
+        any_diag = any_diag || it.diag;
+    }
+    DeviceScope ds(ctx->device);
+    int rc = ensure_workers(ctx);
+    if (rc) return rc;
+    for (int i = 0; i < n && steps > 0 && rc == CHY_OK; ++i) rc = ensure_refine(items[i].grid);
+    if (rc) return rc;
+    if (any_diag && ctx->diag_cap < n) {
+        drop_graphs_of(ctx->batch_graphs, kBatchKeyHead, kBatchKeyPerItem, nullptr);  // they copy into the old staging
+        if (ctx->d_diag) (void)hipFree(ctx->d_diag);
+        if (ctx->h_diag) (void)hipHostFree(ctx->h_diag);
+        ctx->d_diag = ctx->h_diag = nullptr;
+        ctx->diag_cap = 0;
+        if (hipMalloc(&ctx->d_diag, sizeof(double) * 3 * n) != hipSuccess) return CHY_ENOMEM;
+        if (hipHostMalloc(&ctx->h_diag, sizeof(double) * 3 * n) != hipSuccess) return CHY_ENOMEM;
+        ctx->diag_cap = n;
+    }
+    // one graph launch for the whole batch unless stage timing is on, graphs
+    // are off or the caller is capturing the context's stream
+    if (ctx->graphs && !ctx->prof.on && !capturing(ctx->stream))
+        rc = run_batch_graph(ctx, items, n, steps, any_diag);
+    else
+        rc = issue_batch(ctx, items, n, steps);
     if (rc || !any_diag) return rc;
     hipError_t e = hipMemcpyAsync(ctx->h_diag, ctx->d_diag, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
                                   ctx->stream);

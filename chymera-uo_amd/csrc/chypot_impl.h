@@ -200,6 +200,10 @@ struct chy_ctx {
         hipEvent_t done = nullptr;
     };
     std::vector<GroupGraph> group_graphs;
+    // chy_solve_batch: captured graphs of whole batches (fork, groups, join),
+    // keyed on every item, the worker count, the stream and the diagnostics
+    // staging, most recently used last
+    std::vector<GroupGraph> batch_graphs;
 };
 
 namespace chy {

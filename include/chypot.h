@@ -177,7 +177,8 @@ int chy_pot3(chy_grid *g, int npoint, const double *rho_d, double *phi_d, double
  * same set; a call with different arguments re-captures. */
 int chy_solve(chy_grid *g, int maxtrm, int isym, double redge, int npoint, const double *rho_d,
               double *phi_d, double *coef_d, int coef_mode, chy_diag *diag);
-/* Enable (default) or disable graph replay of chy_solve on this context. */
+/* Enable (default) or disable graph replay of chy_solve and chy_solve_batch
+ * on this context. */
 int chy_ctx_graphs(chy_ctx *ctx, int enable);
 
 /* ---- batched solves (no reference counterpart: the reference solves one
@@ -191,7 +192,12 @@ int chy_ctx_graphs(chy_ctx *ctx, int enable);
  * together, up to 32 per launch of each kernel; other items overlap on the
  * worker streams.  Each item's results are bitwise those of chy_solve on the
  * same arguments.  If any item has a diag pointer the call waits and fills
- * every such diag.  Every item is checked before anything is launched. */
+ * every such diag.  Every item is checked before anything is launched.
+ * Unless stage timing is on, graphs are off or the caller is capturing the
+ * context's stream, the whole batch (fork, groups, join) is captured into one
+ * HIP graph the first time a given batch runs (same grids, arrays, arguments,
+ * worker count and stream) and replayed as one launch by later calls; a
+ * change to a grid's solver or tuning drops the graphs that use it. */
 typedef struct {
     chy_grid *grid;
     const double *rho_d;
