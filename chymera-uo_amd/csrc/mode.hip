@@ -741,33 +741,55 @@ int launch_rhs_edges_f(const chy_grid *g, const Fields &F, hipStream_t s)
 // The azimuthal DFT diagonalises the 7-point operator exactly (its L
 // difference becomes 2 (cos m DTHETA - 1) C in BM_m, the axis coupling the
 // ((-1)^m - 1) AMMIN term), so this is the residual of the physical-space
-// operator, mode by mode.  One thread per (q, j); a sliding window down k.
+// operator, mode by mode.  One thread per (q, j) and block of RROWS rows k:
+// blockIdx.x is the row block, so the blocks in flight at once cover
+// consecutive rows of few planes (one block walking every row of its plane
+// kept a separate stream open in each of thousands of planes); every load of
+// the block's rows is issued before the first is used.
+constexpr int RROWS = 8;
 __global__ __launch_bounds__(256) void k_spec_residual(ModeArgs a, const Fields F, const double *bn)
 {
     const int q = blockIdx.y, fi = blockIdx.z;
+    const int nkb = (a.NZ + RROWS - 1) / RROWS;  // row blocks per column block
+    const int k0 = (int)(blockIdx.x % nkb) * RROWS;
     const int lane = threadIdx.x & 63;
-    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int j = (int)(blockIdx.x / nkb) * 256 + threadIdx.x;
     // every lane of a wavefront stays to the end (its neighbours read its
     // values); lanes past M compute on clamped columns and store nothing
     const bool live = j < a.M;
     const int jc = live ? j : a.M - 1;
     const int m = q <= a.N ? q : q - a.N;
+    const int NZ = a.NZ;
     const double *B = static_cast<const double *>(F.spec[fi]) + (size_t)q * a.K * a.J + jc;
     const double *X = static_cast<const double *>(F.aux[fi]) + (size_t)q * a.PP * a.JP + jc;
     float *Rf = static_cast<float *>(F.yh[fi]) + (size_t)q * a.K * a.J + jc;
     const double amj = a.am[jc], cmj = a.cm[jc], bmj = a.bm[(size_t)m * a.JP + jc];  // AM(1) = CM(M) = 0
     // the wavefront's edge lanes take their outer neighbour from memory
     const bool jl = j > 0, jr = j + 1 < a.M;
-    double xm = 0.0, x0 = X[0];
-    for (int k = 0; k < a.NZ; ++k) {
-        const double xp = k + 1 < a.NZ ? X[(size_t)(k + 1) * a.JP] : 0.0;
-        double xl = __shfl_up(x0, 1, 64), xrt = __shfl_down(x0, 1, 64);
-        if (lane == 0) xl = jl ? X[(size_t)k * a.JP - 1] : 0.0;
-        if (lane == 63) xrt = jr ? X[(size_t)k * a.JP + 1] : 0.0;
-        const double ax = a.an_dev[k] * xm + (bn[k] + bmj) * x0 + a.cn_dev[k] * xp + amj * xl + cmj * xrt;
-        if (live) Rf[(size_t)k * a.J] = (float)(B[(size_t)k * a.J] - ax);
-        xm = x0;
-        x0 = xp;
+    // X rows k0-1 .. k0+RROWS, B rows k0 .. k0+RROWS-1, the edge lanes' outer neighbours
+    double xs[RROWS + 2], bs[RROWS], el[RROWS], er[RROWS];
+#pragma unroll
+    for (int i = 0; i < RROWS + 2; ++i) {
+        const int k = k0 - 1 + i;
+        xs[i] = (k >= 0 && k < NZ) ? X[(size_t)k * a.JP] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < RROWS; ++i) {
+        const int k = k0 + i;
+        bs[i] = k < NZ ? B[(size_t)k * a.J] : 0.0;
+        el[i] = (lane == 0 && jl && k < NZ) ? X[(size_t)k * a.JP - 1] : 0.0;
+        er[i] = (lane == 63 && jr && k < NZ) ? X[(size_t)k * a.JP + 1] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < RROWS; ++i) {
+        const int k = k0 + i;
+        if (k >= NZ) break;  // the same k for the whole block
+        const double xm = xs[i], xc = xs[i + 1], xp = xs[i + 2];
+        double xl = __shfl_up(xc, 1, 64), xrt = __shfl_down(xc, 1, 64);
+        if (lane == 0) xl = el[i];
+        if (lane == 63) xrt = er[i];
+        const double ax = a.an_dev[k] * xm + (bn[k] + bmj) * xc + a.cn_dev[k] * xp + amj * xl + cmj * xrt;
+        if (live) Rf[(size_t)k * a.J] = (float)(bs[i] - ax);
     }
 }
 
@@ -1253,7 +1275,7 @@ int launch_spec_residual_f(const chy_grid *g, const Fields &F, hipStream_t s)
     ModeArgs a = make_args(g);
     a.an_dev = g->d.rz_an;
     a.cn_dev = g->d.rz_cn;
-    const dim3 grid((g->M + 255) / 256, g->L, F.nf);
+    const dim3 grid((unsigned)(((g->NZ + RROWS - 1) / RROWS) * ((g->M + 255) / 256)), g->L, F.nf);
     hipLaunchKernelGGL(k_spec_residual, grid, dim3(256), 0, s, a, F, g->d.rz_bn);
     return (int)hipGetLastError();
 }

@@ -185,6 +185,27 @@ def test_batched_refined_equals_single(ctx):
         pg.close()
 
 
+@pytest.mark.parametrize("jkl", [(300, 16, 32), (600, 8, 24)])
+def test_refined_wide_grids_against_oracle(ctx, jkl):
+    """JMAX above 257: the residual's columns span two or three blocks of
+    256 (the last one partial), the r solves take more rows per lane; LMAX 24
+    adds a mixed-radix transform.  Two steps against the oracle's binary64
+    solve."""
+    import pyoracle as po
+    po.use_extended(False)
+    g = fields.unit_grid(*jkl)
+    rho = fields.density_spiral_torus(g, seed=9)
+    print(f"[gpu] refined wide grid {g.name}", flush=True)
+    want = po.solve(g, rho)["phi"]
+    pg = chypot.PreparedGrid(ctx, g)
+    rho_d = chypot.to_device(rho)
+    phi, _, _, rel = refined(ctx, pg, g, rho_d, dict(maxtrm=10, isym=2, redge=0.0, npoint=8), 2)
+    pg.close()
+    err = rel_max(phi, want)
+    print(f"[gpu] refined wide grid {g.name}: corrections {rel}, vs oracle {err:.2e}", flush=True)
+    assert err < 1e-10
+
+
 def test_refined_needs_kmax_at_most_128(ctx):
     g = fields.unit_grid(16, 256, 16)
     pg = chypot.PreparedGrid(ctx, g)
