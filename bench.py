@@ -234,7 +234,7 @@ def serve_run(ctx, args, tune_table=None):
     out = serve.latency_summary(done)
     out.update({"offered_rate_per_s": args.serve_rate, "seconds": args.serve, "budget_gib": args.serve_budget_gib,
                 "completed_per_s": len(done) / wall, "refine_steps": args.refine_steps,
-                "mix": "60% 128x64x128, 30% 256x128x256, 10% 512x256x512 (refined where KMAX <= 128)"})
+                "mix": "60% 128x64x128, 30% 256x128x256, 10% 512x256x512 (refined where KMAX <= 256: all)"})
     return out
 
 

@@ -20,7 +20,7 @@ a grid shape and a device-resident density; the server
   * records per-request latency from arrival to the host seeing completion.
 
 refine_steps > 0 serves the refined solve (binary32 correction solves on the
-FP32 matrix cores, binary64 residuals) on grids it supports (KMAX <= 128);
+FP32 matrix cores, binary64 residuals) on grids it supports (KMAX <= 256);
 other grids get the plain binary64 solve.  Each request records its path.
 A chypot.autotune table (tune_table) sets the launch shapes of every grid
 the server prepares.
@@ -61,8 +61,8 @@ def field_bytes(shape) -> int:
 
 def refine_ok(shape) -> bool:
     """The refined solve's z transforms keep a slot's weights in registers
-    for KMAX-1 <= 127."""
-    return shape[1] <= 128
+    for KMAX-1 <= 255."""
+    return shape[1] <= 256
 
 
 class Slot:

@@ -618,6 +618,18 @@ void launch_zgemm(const chy_grid *g, const ModeArgs &a, const Fields &F, hipStre
         else
             hipLaunchKernelGGL((k_zgemm_reg<FWD, 64, PCT, MB, TI, TO, AT, EPI>), dim3(blocks), dim3(256), 0, s, a, F,
                                col_tiles, ntiles);
+    } else if (g->PP == 256 && sizeof(AT) == 4) {
+        // binary32 products of KMAX = 256 (the refined solve): 16 wavefronts
+        // of 16 rows, 16-column tiles (wider tiles spill the 64 A operands'
+        // neighbours out of the 128 registers a wavefront has), one block per CU
+        if constexpr (sizeof(AT) == 4) {
+            constexpr int PCT2 = 16;
+            const int col_tiles = ((FWD ? g->JP : g->M) + PCT2 - 1) / PCT2;
+            const int ntiles = col_tiles * g->L * F.nf;
+            const int blocks = std::min(ntiles, g->ncu);
+            hipLaunchKernelGGL((k_zgemm_reg<FWD, 256, PCT2, 1, TI, TO, AT, EPI>), dim3(blocks), dim3(1024), 0, s, a,
+                               F, col_tiles, ntiles);
+        }
     } else if (FWD) {
         dim3 gf((g->JP + CT - 1) / CT, (g->PP + RT - 1) / RT, g->L * F.nf);
         hipLaunchKernelGGL((k_zgemm<true, TO>), gf, dim3(GEMM_THREADS), 0, s, a, F);
@@ -682,7 +694,7 @@ int launch_mode_solve(const chy_grid *g, bool rhs_edges, hipStream_t s, Profiler
     return launch_mode_solve_f(g, F, rhs_edges, s, prof);
 }
 
-int refine_supported(const chy_grid *g) { return g->PP == 128 || g->PP == 64; }
+int refine_supported(const chy_grid *g) { return g->PP == 256 || g->PP == 128 || g->PP == 64; }
 
 int launch_mode_solve32_f(const chy_grid *g, const Fields &Ff, const Fields &Fi, int m_fp64, bool rhs64, int epi,
                           int stat_step, hipStream_t s, Profiler *prof)

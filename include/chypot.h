@@ -233,7 +233,7 @@ int chy_pot3_host(chy_grid *g, int npoint, const double *rho_h, double *phi_h, d
  * systems (pot/pot3.f:203-231) at the binary64 solution so far, rounded to
  * binary32, its solution added in binary64; then the inverse transform,
  * POT3's ghost cells and ZAXPHI as in chy_solve.  Two steps put PHI within
- * ~1e-11 of chy_solve's at 256x128x256.  KMAX <= 128 (CHY_EUNSUPPORTED
+ * ~1e-11 of chy_solve's at 256x128x256.  KMAX <= 256 (CHY_EUNSUPPORTED
  * otherwise).  rel_corr (host, may be NULL) receives, per step, the largest
  * correction relative to the largest |solution| after it (1 for the first
  * step).  Waits when diag or rel_corr is non-NULL. */

@@ -206,8 +206,28 @@ def test_refined_wide_grids_against_oracle(ctx, jkl):
     assert err < 1e-10
 
 
-def test_refined_needs_kmax_at_most_128(ctx):
-    g = fields.unit_grid(16, 256, 16)
+@pytest.mark.parametrize("jkl,steps", [((64, 256, 32), 2), ((64, 256, 32), 3), ((130, 256, 48), 2)])
+def test_refined_kmax256_against_oracle(ctx, jkl, steps):
+    """KMAX = 256: the binary32 z products on 16 wavefronts of 16 rows each
+    (16-column tiles); 255 z rows in the residual.  Against the oracle's
+    binary64 solve, every step's correction printed."""
+    import pyoracle as po
+    po.use_extended(False)
+    g = fields.unit_grid(*jkl)
+    rho = fields.density_spiral_torus(g, seed=11)
+    print(f"[gpu] refined KMAX=256 {g.name}, {steps} steps", flush=True)
+    want = po.solve(g, rho)["phi"]
+    pg = chypot.PreparedGrid(ctx, g)
+    rho_d = chypot.to_device(rho)
+    phi, _, _, rel = refined(ctx, pg, g, rho_d, dict(maxtrm=10, isym=2, redge=0.0, npoint=8), steps)
+    pg.close()
+    err = rel_max(phi, want)
+    print(f"[gpu] refined KMAX=256 {g.name}: corrections {rel}, vs oracle {err:.2e}", flush=True)
+    assert err < 1e-10
+
+
+def test_refined_needs_kmax_at_most_256(ctx):
+    g = fields.unit_grid(16, 512, 16)
     pg = chypot.PreparedGrid(ctx, g)
     rho = torch.zeros(pg.field_shape, dtype=torch.float64, device="cuda")
     with pytest.raises(chypot.ChyError) as ei:

@@ -79,9 +79,8 @@ def test_same_shape_requests_share_a_launch(ctx):
 def test_config5_mix_under_a_budget(ctx):
     """The scope's serving configuration: Poisson arrivals over 60%
     128x64x128, 30% 256x128x256 and 10% 512x256x512 under an HBM budget,
-    refined where the grid allows it (KMAX <= 128), plain FP64 for the
-    512x256x512 requests; every request alone equals the same request in the
-    mix, bit for bit, and the books balance."""
+    every request by the refined solve (KMAX <= 256); every request alone
+    equals the same request in the mix, bit for bit, and the books balance."""
     mix = [((128, 64, 128), 0.6), ((256, 128, 256), 0.3), ((512, 256, 512), 0.1)]
     dens = {shape: serve.density_pool(shape, 2, seed=17, on_device=True) for shape, _ in mix}
     torch.cuda.synchronize()
@@ -103,7 +102,7 @@ def test_config5_mix_under_a_budget(ctx):
     print(f"[gpu] config-5 serving: {summary}, peak {peak / 2**30:.2f} GiB", flush=True)
     assert len(done) == len(reqs)
     assert peak <= 24 << 30
-    assert summary["paths"].get("plain", 0) == sum(r.shape == (512, 256, 512) for r in reqs)
+    assert summary["paths"].get("refined", 0) == len(reqs)
     mixed = {r.rid: r.phi for r in done}
     seen = set()
     for r in reqs:
