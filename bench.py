@@ -185,20 +185,33 @@ def hetero_batch(ctx, n, refine_steps, reps=3):
 
     import chypot
     from chypot import fields
-    jobs, points, shapes = [], 0, set()
+    from chypot import autotune
+    jobs, points, shapes = [], 0, {}
     for i in range(n):
         g, rho_d = fields.random_field_device(1000 + i)
         pg = chypot.PreparedGrid(ctx, g)
-        shapes.add((g.jmax, g.kmax, g.lmax))
+        key = (g.jmax, g.kmax, g.lmax)
+        shapes.setdefault(key, [g, 0])[1] += 1
         jobs.append(dict(grid=pg, rho=rho_d, phi=torch.zeros_like(rho_d),
                          coef=torch.zeros(pg.coef_shape, dtype=torch.float64, device="cuda"),
                          coef_mode=chypot.COEF_K2))
         points += g.jmax * g.kmax * g.lmax
+    # launch shapes of every grid class in the set, by the autotuner, at the
+    # batch sizes the set has (1 and its largest group)
+    t_tune = time.perf_counter()
+    most = max(c for _, c in shapes.values())
+    table = autotune.tune(ctx, [g for g, _ in shapes.values()], batch_sizes=tuple(sorted({1, most})),
+                          refine_steps=refine_steps, reps=5)
+    for j in jobs:
+        autotune.apply(j["grid"], table)
+    tune_s = time.perf_counter() - t_tune
     # many small groups: more worker streams overlap more of them (16 measured
     # 5-6% faster than 8 on this set)
     workers = 16
     ctx.workers(workers)
-    out = {"fields": n, "distinct_grids": len(shapes), "refine_steps": refine_steps, "workers": workers}
+    out = {"fields": n, "distinct_grids": len(shapes), "refine_steps": refine_steps, "workers": workers,
+           "autotune": {"seconds": round(tune_s, 2), "batch_sizes": sorted({1, most}),
+                        "one_kernel_per_plane_classes": sum(e["refine_fused"] > 0 for e in table.values())}}
     for label, steps in (("refined", refine_steps), ("plain_fp64", 0)):
         ctx.solve_batch(jobs, refine_steps=steps)
         torch.cuda.synchronize()

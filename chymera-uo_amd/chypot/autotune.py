@@ -22,7 +22,7 @@ only), so tuning never changes what a caller gets beyond that.
 The result is a table {"JxKxL": {"cr_lanes": .., "refine_fused": ..,
 "timings_ms": {...}}} that apply() puts on prepared grids and save()/load()
 keep as JSON.  chypot.serve.Server applies a table to the grids it prepares.
-measured_table() is the table this tuner made on an MI355X for nine grid
+measured_table() is the table this tuner made on an MI355X for 30 grid
 classes.
 """
 from __future__ import annotations
@@ -133,9 +133,12 @@ def apply(pg: PreparedGrid, table: dict) -> bool:
 
 
 def measured_table() -> dict:
-    """The table this tuner produced on an MI355X for nine grid classes
-    (tuned_mi355x.json beside this module): batch sizes 1, 8, 32 (1, 8 for
-    JMAX = 512), refined solves of 2 steps, medians of 3 calls."""
+    """The table this tuner produced on an MI355X (tuned_mi355x.json beside
+    this module), refined solves of 2 steps: nine grid classes with batch
+    sizes 1, 8, 32 (1, 8 for JMAX = 512), medians of 3 calls; and the other
+    21 classes of the heterogeneous set JMAX in {64,128,256}, KMAX in
+    {32,64,128}, LMAX in {64,128,256} with batch sizes 1 and 4, medians of 5
+    calls."""
     import pathlib
     return load(pathlib.Path(__file__).with_name("tuned_mi355x.json"))
 
