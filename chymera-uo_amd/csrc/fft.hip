@@ -134,8 +134,6 @@ struct FftArgs {
     double sf1, pi;
     double ys;            // factor on the interior lanes' spectrum (4 pi G)
     int ghosts;           // inverse: also write POT3's axis/midplane ghost cells
-    unsigned long long *stat;  // inverse with accumulation: max |dPHI|, max |PHI| (as bits), or null
-    int stat_step;             // refinement step whose statistics pair stat points at
     double *zsamp;        // inverse: [KMAX][kZaxSamples][L] copy of PHI(J=2..6,K,L) for ZAXPHI, or null
 };
 
@@ -166,7 +164,6 @@ __device__ __forceinline__ void bind(FftArgs &a, const Fields &F, int f)
     if (a.spec) a.spec = F.spec[f];
     if (a.mom) a.mom = F.mom[f];
     if (a.zsamp) a.zsamp = F.zsamp[f];
-    if (a.stat) a.stat = F.rstat[f] + 2 * a.stat_step;
     a.coef = a.coef_mode == CHY_COEF_NONE ? nullptr : F.coef[f];
 }
 
@@ -371,10 +368,8 @@ __global__ __launch_bounds__(NL *N2) void k_fft_forward(FftArgs a, const Fields 
     forward_outputs<TS>(a, lds, tw, N, NL, N2, lane, sub, j, k, interior, base, plane);
 }
 
-// TS: storage type of the spectrum.  ACC: add the result to PHI (the
-// correction of a refinement step; a.stat collects max |dPHI| and max |PHI|)
-// instead of writing it.
-template <int N1, int N2, int NL, typename TS, bool ACC>
+// TS: storage type of the spectrum.
+template <int N1, int N2, int NL, typename TS>
 __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a, const Fields F)
 {
     bind(a, F, blockIdx.y);
@@ -450,62 +445,13 @@ __global__ __launch_bounds__(NL *N2) void k_fft_inverse(FftArgs a, const Fields 
     const size_t plane = (size_t)a.JS * a.KS;
     const size_t base = (size_t)(j + 1) + (size_t)a.JS * (size_t)(k + 1);
     const bool axis = a.ghosts && j == 0, mid = a.ghosts && k == 0;
-    double dmax = 0.0, pmax = 0.0;
-    if (active) {
-    if (ACC) {  // the old values first, all at once
-        double old[R2][N2][2];
-#pragma unroll
-        for (int r = 0; r < R2; ++r)
-#pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) {
-                const int n = sub + N2 * r + N1 * k2;
-                old[r][k2][0] = a.phi_out[base + plane * (size_t)(2 * n)];
-                old[r][k2][1] = a.phi_out[base + plane * (size_t)(2 * n + 1)];
-            }
-#pragma unroll
-        for (int r = 0; r < R2; ++r)
-#pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) {
-                const double dx = u[r][k2].x, dy = u[r][k2].y;
-                dmax = fmax(dmax, fmax(fabs(dx), fabs(dy)));
-                u[r][k2].x = old[r][k2][0] + dx;
-                u[r][k2].y = old[r][k2][1] + dy;
-                pmax = fmax(pmax, fmax(fabs(u[r][k2].x), fabs(u[r][k2].y)));
-            }
-    }
+    if (!active) return;
 #pragma unroll
     for (int r = 0; r < R2; ++r) {
         const int k1 = sub + N2 * r;
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) {
-            store_pair(a, k1 + N1 * k2, u[r][k2].x, u[r][k2].y, j, k, base, plane, axis, mid, !ACC, N, L);
-        }
-    }
-    }  // active
-    if (ACC && a.stat) {
-        // wavefront maxima, block maxima through LDS, then at most one atomic
-        // per block and statistic, skipped when the running maximum is already
-        // larger: every block raising the same two words one wavefront at a
-        // time serialises at the cache.  Non-negative doubles order like their
-        // bit patterns.
-        __shared__ double red[2][NL * N2 / 64];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
-            pmax = fmax(pmax, __shfl_xor(pmax, off, 64));
-        }
-        if ((tid & 63) == 0) {
-            red[0][tid >> 6] = dmax;
-            red[1][tid >> 6] = pmax;
-        }
-        __syncthreads();
-        if (tid < 2) {
-            double m = red[tid][0];
-            for (int w = 1; w < NL * N2 / 64; ++w) m = fmax(m, red[tid][w]);
-            const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
-            unsigned long long *dst = &a.stat[tid];
-            if (bits > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(dst, bits);
-        }
+        for (int k2 = 0; k2 < N2; ++k2)
+            store_pair(a, k1 + N1 * k2, u[r][k2].x, u[r][k2].y, j, k, base, plane, axis, mid, true, N, L);
     }
 }
 
@@ -714,7 +660,7 @@ int launch_one(const FftArgs &a, const Fields &F, hipStream_t s)
     const int blocks = (chunks + kXcds - 1) / kXcds * kXcds;
     const dim3 grid(blocks, F.nf), block(NL * N2);
     if (KIND == FWD64) hipLaunchKernelGGL((k_fft_forward<N1, N2, NL, double, double>), grid, block, 0, s, a, F);
-    if (KIND == INV64) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, double, false>), grid, block, 0, s, a, F);
+    if (KIND == INV64) hipLaunchKernelGGL((k_fft_inverse<N1, N2, NL, double>), grid, block, 0, s, a, F);
     return (int)hipGetLastError();
 }
 
