@@ -356,3 +356,35 @@ def test_worker_counts_graph_caches_and_teardown(ctx):
     x = torch.ones(4096, dtype=torch.float64, device="cuda")
     assert float(x.sum()) == 4096.0
     torch.cuda.synchronize()
+
+
+def test_batch_graph_replays_with_diagnostics(ctx):
+    """A batch with diagnostics replayed from its captured graph several
+    times (the per-item diagnostics go through a copy node of the graph),
+    with eager work between the replays: every replay's PHI and diagnostics
+    equal the eager batch's."""
+    print("[gpu] batch graph replays with diagnostics", flush=True)
+    jobs = [_own_grid_job(ctx, 900 + i, (32, 16, 64)) for i in range(3)]
+    jobs += [_own_grid_job(ctx, 910, (24, 16, 40)), _own_grid_job(ctx, 911, (16, 32, 32))]
+    try:
+        ctx.graphs(False)
+        want_d = ctx.solve_batch(jobs, diag=True)
+        want = [jb["phi"].clone() for jb in jobs]
+        ctx.graphs(True)
+        for rep in range(4):
+            for jb in jobs:
+                jb["phi"].zero_()
+            got_d = ctx.solve_batch(jobs, diag=True)
+            for i, (jb, w, dw, dg) in enumerate(zip(jobs, want, want_d, got_d)):
+                assert torch.equal(jb["phi"], w), (rep, i)
+                assert (dg.tmass, dg.phichk, dg.klocat) == (dw.tmass, dw.phichk, dw.klocat), (rep, i)
+            # eager work between the replays, into other buffers
+            ctx.graphs(False)
+            other = [dict(jb, phi=torch.zeros_like(jb["phi"])) for jb in jobs]
+            ctx.solve_batch(other)
+            ctx.sync()
+            ctx.graphs(True)
+    finally:
+        ctx.graphs(True)
+        for jb in jobs:
+            jb["grid"].close()
