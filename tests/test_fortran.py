@@ -20,7 +20,8 @@ FDIR = ROOT / "chymera-uo_amd" / "fortran"
 REFERENCE_HEADERS = pathlib.Path("/root/reference/headers")
 FLANG = shutil.which("flang")
 
-DROPIN_CASES = ["g16_small", "g16_m4_np5", "g16_isymneg_redge", "g32_redge", "c1_64x32x64"]
+DROPIN_CASES = ["g16_small", "g16_m4_np5", "g16_isymneg_redge", "g32_redge", "c1_64x32x64",
+                "g24x16x40_radix5", "g16x8x46_radix23"]
 
 
 def potdiag(grid_name):
