@@ -613,7 +613,7 @@ const char *chy_strerror(int code)
     switch (code) {
     case CHY_OK: return "success";
     case CHY_EINVAL: return "invalid argument or grid description";
-    case CHY_EUNSUPPORTED: return "unsupported grid (KMAX must be 2^k >= 4, LMAX even and <= 1024 with LMAX/2 23-smooth, JMAX <= 1024, |ISYM| = 2)";
+    case CHY_EUNSUPPORTED: return "unsupported grid (KMAX must be 2^k >= 4, LMAX even and <= 1024 with LMAX/2 23-smooth, JMAX <= 4096, |ISYM| = 2)";
     case CHY_ENOMEM: return "out of memory";
     case CHY_EDEVICE: return "HIP runtime error";
     case CHY_ESTATE: return "call out of order";

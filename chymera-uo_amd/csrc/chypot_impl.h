@@ -100,6 +100,7 @@ struct DeviceGrid {
     double *am = nullptr;        // [JP]
     double *cm = nullptr;        // [JP]
     double *bm = nullptr;        // [(N+1)][JP]
+    double *rinv = nullptr;      // [(N+1)][NZ][JP] reciprocal Thomas pivots of T_m + lambda_p (JMAX-1 > 1024 only)
     // the z operator for the refinement's spectral residuals: AN (AN(1)=0),
     // CN (CN(NZ)=0; CNMAX separate), BN
     double *rz_an = nullptr;     // [NZ]
@@ -310,6 +311,8 @@ int launch_fft_inverse_f(const chy_grid *g, const Fields &F, hipStream_t s);
 // LMAX the transforms take: even, at most 1024, LMAX/2 with no prime factor
 // above 23 (powers of two by the specialised kernels, the rest mixed-radix)
 bool fft_size_ok(int L);
+// the reciprocal pivots of the r solves by the Thomas algorithm (JMAX-1 > 1024)
+int launch_thomas_pivots(const chy_grid *g, hipStream_t s);
 
 // Mode solves on d.spec; rhs_edges applies the boundary terms of Y first
 // (skipped when the spectral BDYGEN has applied them).

@@ -559,6 +559,62 @@ __global__ __launch_bounds__(256) void k_rsolve(ModeArgs a, const Fields F, int 
     if (two) store_rows<R>(y1 + f, x1);
 }
 
+// r solves of grids whose systems do not fit a wavefront's registers
+// (JMAX-1 > 1024): the Thomas algorithm, one thread per (m, p) with both
+// right-hand sides, down and back up along r.  The systems are diagonally
+// dominant (|BM + lambda_p| >= |AM| + |CM|), so no pivoting is needed.  The
+// reciprocal pivots 1/(BM_m(j) + lambda_p - AM(j) g(j-1)), g(j) = CM(j) times
+// that, depend on the grid alone and are tabulated once per grid in rinv
+// ([m][p][j], binary64; k_thomas_pivots).
+__global__ __launch_bounds__(256) void k_thomas_pivots(ModeArgs a, double *rinv)
+{
+    const int sys = blockIdx.x * 256 + threadIdx.x;
+    if (sys >= (a.N + 1) * a.NZ) return;
+    const int m = sys / a.NZ, p = sys % a.NZ;
+    const double lam = a.lam[p];
+    const double *bm = a.bm + (size_t)m * a.JP;
+    double *out = rinv + (size_t)sys * a.JP;
+    double g = 0.0;
+    for (int j = 0; j < a.M; ++j) {
+        const double inv = 1.0 / ((bm[j] + lam) - a.am[j] * g);
+        out[j] = inv;
+        g = a.cm[j] * inv;
+    }
+}
+
+template <typename TS, typename AT>
+__global__ __launch_bounds__(256) void k_rsolve_thomas(ModeArgs a, const Fields F, int m0, int m1,
+                                                       const double *rinv)
+{
+    const int sys = blockIdx.x * 256 + threadIdx.x;
+    if (sys >= (m1 - m0) * a.NZ) return;
+    const int m = m0 + sys / a.NZ, p = sys % a.NZ;
+    const bool two = (m >= 1 && m <= a.N - 1);
+    TS *y0 = static_cast<TS *>(F.yh[blockIdx.y]) + ((size_t)m * a.PP + p) * a.JP;
+    TS *y1 = static_cast<TS *>(F.yh[blockIdx.y]) + ((size_t)(a.N + m) * a.PP + p) * a.JP;
+    const double *inv = rinv + ((size_t)m * a.NZ + p) * a.JP;
+    AT d0 = (AT)0, d1 = (AT)0;
+    for (int j = 0; j < a.M; ++j) {
+        const AT iv = (AT)inv[j], aj = (AT)a.am[j];
+        d0 = ((AT)y0[j] - aj * d0) * iv;
+        y0[j] = (TS)d0;
+        if (two) {
+            d1 = ((AT)y1[j] - aj * d1) * iv;
+            y1[j] = (TS)d1;
+        }
+    }
+    AT x0 = (AT)0, x1 = (AT)0;
+    for (int j = a.M - 1; j >= 0; --j) {
+        const AT gj = (AT)(a.cm[j] * inv[j]);
+        x0 = (AT)y0[j] - gj * x0;
+        y0[j] = (TS)x0;
+        if (two) {
+            x1 = (AT)y1[j] - gj * x1;
+            y1[j] = (TS)x1;
+        }
+    }
+}
+
 // Y edge terms of pot3.f:212-224 on the spectrum (whose interior lanes the
 // forward transform has already scaled by 4 pi G): the K=KMAX row loses
 // CNMAX times the top boundary lane, the J=JMAX column CMMAX times the outer
@@ -651,7 +707,10 @@ int launch_rsolve(const chy_grid *g, const ModeArgs &a, const Fields &F, hipStre
     case 4: hipLaunchKernelGGL((k_rsolve<4, TS, AT>), grid, block, 0, s, a, F, m0, m1); break;
     case 8: hipLaunchKernelGGL((k_rsolve<8, TS, AT>), grid, block, 0, s, a, F, m0, m1); break;
     case 16: hipLaunchKernelGGL((k_rsolve<16, TS, AT>), grid, block, 0, s, a, F, m0, m1); break;
-    default: return (int)hipErrorInvalidValue;
+    default:
+        if (!g->d.rinv) return (int)hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_rsolve_thomas<TS, AT>), dim3((nsys + 255) / 256, F.nf), block, 0, s, a, F, m0, m1,
+                           g->d.rinv);
     }
     return (int)hipGetLastError();
 }
@@ -1289,6 +1348,14 @@ int launch_spec_residual_f(const chy_grid *g, const Fields &F, hipStream_t s)
     a.cn_dev = g->d.rz_cn;
     const dim3 grid((unsigned)(((g->NZ + RROWS - 1) / RROWS) * ((g->M + 255) / 256)), g->L, F.nf);
     hipLaunchKernelGGL(k_spec_residual, grid, dim3(256), 0, s, a, F, g->d.rz_bn);
+    return (int)hipGetLastError();
+}
+
+int launch_thomas_pivots(const chy_grid *g, hipStream_t s)
+{
+    ModeArgs a = make_args(g);
+    const int n = (g->N + 1) * g->NZ;
+    hipLaunchKernelGGL(k_thomas_pivots, dim3((n + 255) / 256), dim3(256), 0, s, a, g->d.rinv);
     return (int)hipGetLastError();
 }
 

@@ -219,7 +219,7 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
     if (!is_pow2(K) || K < 4) return CHY_EUNSUPPORTED;
     // LMAX/2 may have prime factors up to 23, as in pot/fft.f:111-126
     if (!fft_size_ok(L)) return CHY_EUNSUPPORTED;
-    if (J > 1024) return CHY_EUNSUPPORTED;
+    if (J > 4096) return CHY_EUNSUPPORTED;
     if (std::abs(isym) != 2) return CHY_EUNSUPPORTED;
     if (!(desc->dtheta > 0.0) || !std::isfinite(desc->pi) || !std::isfinite(desc->grav))
         return CHY_EINVAL;
@@ -497,6 +497,17 @@ int build_grid(chy_grid *g, const chy_grid_desc *desc, int isym)
     CHY_TRY(reserve(g, &d.dscal, (size_t)4 + K + 2));
     CHY_TRY(reserve(g, &d.ticket, 1));
     CHY_TRY(reserve(g, &d.zsamp, (size_t)K * kZaxSamples * L));
+    if (JP > 1024) {
+        // r systems longer than a wavefront holds: the Thomas algorithm, its
+        // pivots tabulated once here (on the context's stream, waited for)
+        CHY_TRY(reserve(g, &d.rinv, (size_t)(N + 1) * NZ * JP));
+        hipError_t e = (hipError_t)launch_thomas_pivots(g, g->ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+        if (e != hipSuccess) {
+            set_device_error(e, "r-solve pivots");
+            return CHY_EDEVICE;
+        }
+    }
     return CHY_OK;
 }
 

@@ -47,7 +47,7 @@ enum {
     CHY_EUNSUPPORTED = 2, /* grid the solver cannot take: KMAX not 2^k (BLKTRI's own
                              precondition, pot/blktri.f:29-31), LMAX odd or above
                              1024, LMAX/2 with a prime factor above 23 (FFT's own
-                             limit, pot/fft.f:111-126), JMAX above 1024,
+                             limit, pot/fft.f:111-126), JMAX above 4096,
                              |ISYM| != 2 */
     CHY_ENOMEM = 3,       /* device or host allocation failed */
     CHY_EDEVICE = 4,      /* HIP runtime error */

@@ -269,15 +269,16 @@ def test_config2_eigenbasis_against_exact(ctx):
                                       ((16, 8, 25), chypot.CHY_EUNSUPPORTED),
                                       ((16, 8, 2048), chypot.CHY_EUNSUPPORTED),
                                       ((16, 8, 1026), chypot.CHY_EUNSUPPORTED),
-                                      ((1040, 8, 16), chypot.CHY_EUNSUPPORTED),
+                                      ((4100, 8, 16), chypot.CHY_EUNSUPPORTED),
                                       ((2, 8, 16), chypot.CHY_EINVAL)])
 def test_rejected_grids(ctx, jkl, code):
     """Grids the drop-in refuses with a code instead of solving wrongly:
     KMAX not 2^k (BLKTRI's own precondition, pot/blktri.f:21-31); LMAX/2
     with a prime factor above 23 (58 = 2 x 29: the reference's FFT stops on
     it too, pot/fft.f:111-126); odd LMAX (REALTR needs LMAX = 2N); LMAX above
-    1024 (the transforms keep a lane's spectrum in LDS); JMAX above 1024,
-    where the r solves would need more than 16 rows per lane; JMAX < 3."""
+    1024 (the transforms keep a lane's spectrum in LDS); JMAX above 4096
+    (the r solves' pivot table and padding grow with the next power of two);
+    JMAX < 3."""
     g = fields.unit_grid(*jkl)
     with pytest.raises(chypot.ChyError) as ei:
         chypot.PreparedGrid(ctx, g)
@@ -421,7 +422,8 @@ def test_128x64x128_against_reference(ctx):
 
 @pytest.mark.parametrize("jkl", [(32, 256, 32), (512, 16, 32), (16, 8, 512), (40, 8, 16), (200, 32, 64),
                                  (20, 4, 8), (64, 64, 2), (1024, 8, 16), (16, 8, 1024), (16, 512, 16),
-                                 (128, 64, 192), (64, 32, 768), (16, 8, 1000), (16, 8, 6), (24, 8, 18)])
+                                 (128, 64, 192), (64, 32, 768), (16, 8, 1000), (16, 8, 6), (24, 8, 18),
+                                 (1100, 8, 16), (2050, 16, 8)])
 def test_grid_shapes_against_oracle(ctx, jkl):
     """Shapes that select the other kernel variants: KMAX=256 (general z
     GEMM, eight reduction levels), JMAX=512 (eight r rows per lane), LMAX=512
@@ -430,7 +432,8 @@ def test_grid_shapes_against_oracle(ctx, jkl):
     each direction: JMAX=1024 (sixteen r rows per lane: z eigenbasis only),
     LMAX=1024, KMAX=512; mixed-radix transforms: LMAX/2 = 96 (one pass of
     16, 2 and 3), 384 and 500 (4 lanes per block), 3 and 9 (no spectral
-    boundary lanes: LMAX/2 <= MAXTRM).
+    boundary lanes: LMAX/2 <= MAXTRM); JMAX 1100 and 2050, whose r systems
+    are solved by the Thomas algorithm (one thread per system).
 
     Cyclic reduction (default where the grid allows it): against the oracle's
     POT3 by the same reduction, which is the reference's BLKTRI bit for bit;

@@ -185,7 +185,7 @@ def test_batched_refined_equals_single(ctx):
         pg.close()
 
 
-@pytest.mark.parametrize("jkl", [(300, 16, 32), (600, 8, 24)])
+@pytest.mark.parametrize("jkl", [(300, 16, 32), (600, 8, 24), (1500, 8, 16)])
 def test_refined_wide_grids_against_oracle(ctx, jkl):
     """JMAX above 257: the residual's columns span two or three blocks of
     256 (the last one partial), the r solves take more rows per lane; LMAX 24
