@@ -26,7 +26,7 @@ kernel for all of them):
     for both paths;
   * `rooflines`: every stage of both paths in their timed regions (HIP events
     around each stage on the stream the batched launches run on): algorithmic
-    bytes per field (the scope table's figures: forward transform 134.2 MB,
+    bytes per field (each input read and each output written once: forward transform 134.2 MB,
     cyclic reduction 133.4 MB, inverse 135.8 MB at 256x128x256) over the
     stage time, against 8 TB/s; the z transforms also against the matrix-core
     peak, the cyclic reduction against the FP64 vector peak; where profiles/
@@ -86,7 +86,8 @@ def cr_solves_per_mode(n):
 def stage_work(g, path, steps):
     """Algorithmic bytes (and flops, with the peak they are priced against)
     per field of one solve, per stage, summed over the stage's launches in
-    that solve.  Transforms and the cyclic reduction follow the scope table:
+    that solve.  Transforms and the cyclic reduction, each input read and
+    each output written once:
     forward 2 J K L 8 bytes, reduction J K L 8 + M NZ L 8, inverse J K L 8 +
     (J+2)(K+2) L 8.  The reduction's flops: 10 per r row per solve (8 for a
     tridiagonal elimination and back-substitution, 2 to combine), over the
@@ -177,7 +178,7 @@ def rooflines(stage_times, work, nsolve, fields_per_launch, steps, workload):
 
 
 def hetero_batch(ctx, n, refine_steps, reps=3):
-    """Config 4: one batched call over n independent fields of mixed grids
+    """The heterogeneous batch: one batched call over n independent fields of mixed grids
     (fields.random_field_device for seeds 1000.., generated on the GPU), each
     on its own prepared grid (fields on identical grids share launches);
     fields per second over `reps` calls, refined and plain."""
@@ -227,7 +228,7 @@ def hetero_batch(ctx, n, refine_steps, reps=3):
 
 
 def serve_run(ctx, args, tune_table=None):
-    """Config 5: the serving loop (chypot.serve) over Poisson arrivals,
+    """The served mix: the serving loop (chypot.serve) over Poisson arrivals,
     refined where the grid allows it: throughput and per-request latency
     from arrival to completion."""
     import torch

@@ -197,8 +197,8 @@ def test_batched_launches_match_oracle(ctx):
             jb["grid"].close()
 
 
-def test_config4_64_heterogeneous_fields_refined(ctx):
-    """The scope's batched configuration: 64 independent fields drawn from a
+def test_64_heterogeneous_fields_refined(ctx):
+    """The heterogeneous batch: 64 independent fields drawn from a
     seed (JMAX in {64,128,256}, KMAX in {32,64,128}, LMAX in {64,128,256},
     tori of random centre, radius and index with log-normal noise), each on
     its own grid object, solved in one chy_solve_batch_refined call (fields
@@ -214,7 +214,7 @@ def test_config4_64_heterogeneous_fields_refined(ctx):
                          coef=torch.zeros(pg.coef_shape, dtype=torch.float64, device="cuda"),
                          coef_mode=chypot.COEF_K2))
     shapes = {(j["g"].jmax, j["g"].kmax, j["g"].lmax) for j in jobs}
-    print(f"[gpu] config 4: 64 fields on {len(shapes)} distinct grids", flush=True)
+    print(f"[gpu] heterogeneous batch: 64 fields on {len(shapes)} distinct grids", flush=True)
     ctx.solve_batch(jobs, refine_steps=2)
     ctx.sync()
     worst = 0.0
@@ -229,7 +229,7 @@ def test_config4_64_heterogeneous_fields_refined(ctx):
             j["grid"].solve_refined(j["rho"], alone, steps=2)
             ctx.sync()
             assert torch.equal(alone, j["phi"]), i
-    print(f"[gpu] config 4: worst refined vs plain FP64 {worst:.2e}", flush=True)
+    print(f"[gpu] heterogeneous batch: worst refined vs plain FP64 {worst:.2e}", flush=True)
     assert worst < 1e-10
     small = sorted(range(64), key=lambda i: jobs[i]["g"].jmax * jobs[i]["g"].kmax * jobs[i]["g"].lmax)[:3]
     for i in small:

@@ -26,7 +26,7 @@ def ctx():
 SOLVERS = {"cr": chypot.MODES_CR, "eigen": chypot.MODES_EIGEN}
 # PHI against the reference's FP64 solve: the cyclic reduction follows the
 # reference's own arithmetic (BLKTRI's root tables), the z eigenbasis is
-# closer to the exact solution and within the scope's 1e-12 on these grids
+# closer to the exact solution and within 1e-12 on these grids
 PHI_BAR = {"cr": 1e-13, "eigen": 1e-12}
 
 
@@ -177,7 +177,7 @@ def test_host_entry_points_match_device(ctx):
     assert d.klocat == dev["klocat"]
 
 
-def _config2(ctx, solver):
+def _parity_256(ctx, solver):
     """The 256x128x256 spiral-arm torus: (GPU result, fixture, reference's max |PHI|)."""
     f = np.load(GOLDEN / "solve_large_256x128x256.npz")
     g = fields.unit_grid(256, 128, 256)
@@ -187,8 +187,8 @@ def _config2(ctx, solver):
     return g, rho, res, f, float(f["phi_absmax"])
 
 
-def test_config2_against_reference(ctx):
-    """256x128x256, the scope's parity configuration: the cyclic reduction
+def test_256x128x256_against_reference(ctx):
+    """256x128x256, the spiral-arm torus: the cyclic reduction
     (default solver) within 1e-12 of the reference's FP64 solve, on the
     fixture's K=2 and L=1 planes and the whole-field statistics, and within
     5e-13 of the oracle's POT3 by the same cyclic reduction (bitwise BLKTRI)
@@ -196,7 +196,7 @@ def test_config2_against_reference(ctx):
     parallel cyclic reduction over the lanes -- round differently from
     BLKTRI's serial ones: ~2e-13 here)."""
     import pyoracle as po
-    g, rho, res, f, scale = _config2(ctx, "cr")
+    g, rho, res, f, scale = _parity_256(ctx, "cr")
     phi = res["phi"]
     e_ref = max(rel_max(phi[:, 1, :], f["phi_k2"], scale), rel_max(phi[:, :, 0], f["phi_l1"], scale))
     stats = {}
@@ -249,12 +249,12 @@ def test_cr_lane_shapes_against_oracle(ctx, jkl, lanes):
     assert err < 5e-13
 
 
-def test_config2_eigenbasis_against_exact(ctx):
+def test_256x128x256_eigenbasis_against_exact(ctx):
     """256x128x256 by the z eigenbasis (opt-in fast solver): within 1.5e-12
     of the 80-bit solve of the discrete system; the reference's own FP64
     result is 2.3e-12 from that solve (its TQLRAT root tables), so the
     eigenbasis is 2-4e-12 from the reference."""
-    g, rho, res, f, scale = _config2(ctx, "eigen")
+    g, rho, res, f, scale = _parity_256(ctx, "eigen")
     phi = res["phi"]
     e_ref = max(rel_max(phi[:, 1, :], f["phi_k2"], scale), rel_max(phi[:, :, 0], f["phi_l1"], scale))
     e_ext = max(rel_max(phi[:, 1, :], f["ext_phi_k2"], scale), rel_max(phi[:, :, 0], f["ext_phi_l1"], scale))

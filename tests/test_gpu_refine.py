@@ -104,16 +104,16 @@ def _plain(ctx, pg, rho_d):
     return out
 
 
-@pytest.mark.parametrize("density", ["config1_torus", "spiral"])
+@pytest.mark.parametrize("density", ["small_torus", "spiral"])
 def test_refined_error_per_step_256(ctx, density):
-    """Config 3: the config-1 torus (and the spiral of config 2) at
+    """The small polytropic torus (and the spiral-arm torus) at
     256x128x256.  The error against the plain binary64 solve after each
     refinement step: the first step (binary32 solves) is ~1e-6, the second
     below 1e-10; then it settles at the distance between the exact solution
     of the mode systems and the plain path's cyclic reduction (its
     reference-faithful root tables), ~2e-12."""
     g = fields.unit_grid(256, 128, 256)
-    rho = fields.density_small_torus(g, 1) if density == "config1_torus" else fields.density_spiral_torus(g, 2)
+    rho = fields.density_small_torus(g, 1) if density == "small_torus" else fields.density_spiral_torus(g, 2)
     pg = chypot.PreparedGrid(ctx, g)
     assert pg.refine_m64 == 4
     rho_d = chypot.to_device(rho)

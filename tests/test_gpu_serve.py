@@ -76,8 +76,8 @@ def test_same_shape_requests_share_a_launch(ctx):
     srv.close()
 
 
-def test_config5_mix_under_a_budget(ctx):
-    """The scope's serving configuration: Poisson arrivals over 60%
+def test_served_mix_under_a_budget(ctx):
+    """The served mix: Poisson arrivals over 60%
     128x64x128, 30% 256x128x256 and 10% 512x256x512 under an HBM budget,
     every request by the refined solve (KMAX <= 256); every request alone
     equals the same request in the mix, bit for bit, and the books balance."""
@@ -86,7 +86,7 @@ def test_config5_mix_under_a_budget(ctx):
     torch.cuda.synchronize()
     reqs = serve.poisson_requests(rate=400.0, duration=0.15, mix=mix, densities=dens, seed=5)
     assert any(r.shape == (512, 256, 512) for r in reqs)
-    print(f"[gpu] config-5 mix: {len(reqs)} requests", flush=True)
+    print(f"[gpu] served mix: {len(reqs)} requests", flush=True)
     srv = serve.Server(ctx, hbm_budget=24 << 30, max_batch=16, workers=8, keep_outputs=True, refine_steps=2)
     peak = 0
     orig_step = srv.step
@@ -99,7 +99,7 @@ def test_config5_mix_under_a_budget(ctx):
     done = srv.run(reqs)
     torch.cuda.synchronize()
     summary = serve.latency_summary(done)
-    print(f"[gpu] config-5 serving: {summary}, peak {peak / 2**30:.2f} GiB", flush=True)
+    print(f"[gpu] served mix: {summary}, peak {peak / 2**30:.2f} GiB", flush=True)
     assert len(done) == len(reqs)
     assert peak <= 24 << 30
     assert summary["paths"].get("refined", 0) == len(reqs)
