@@ -116,9 +116,12 @@ def stage_work(g, path, steps):
         m32 = modes // 2
         w["zfwd"] = dict(bytes=modes + m32 + (steps - 1) * 2 * m32, flops=steps * gemm, peak="mfma_f32")
         w["rsolve"] = dict(bytes=steps * 2 * m32, flops=None, peak=None)
-        xb = [m32 + modes] + [m32 + 2 * modes] * (steps - 1)  # X written, then read and written
+        # X written, then read and written; two steps keep their one stored
+        # iterate (a binary32 sum) in binary32
+        xs = m32 if steps == 2 else modes
+        xb = [m32 + (modes if steps == 1 else xs)] + [m32 + 2 * modes] * (steps - 2) + [m32 + xs + modes] * (steps > 1)
         w["zinv"] = dict(bytes=sum(xb), flops=steps * gemm, peak="mfma_f32")
-        w["resid"] = dict(bytes=(steps - 1) * (2 * modes + m32), flops=None, peak=None)
+        w["resid"] = dict(bytes=(steps - 1) * (modes + xs + m32), flops=None, peak=None)
     return w
 
 
@@ -456,7 +459,7 @@ def main():
         # launch shapes of the bench grid for single fields and for the batch
         from chypot import autotune
         t_tune = time.perf_counter()
-        tune_table = autotune.tune(ctx, [g], batch_sizes=(1, B), refine_steps=args.refine_steps, reps=3)
+        tune_table = autotune.tune(ctx, [g], batch_sizes=(1, B), refine_steps=args.refine_steps, reps=5)
         for j in jobs:
             autotune.apply(j["grid"], tune_table)
         ctx.set_stream(stream)

@@ -16,7 +16,7 @@ the fastest:
 
 Timings use synthetic fields of the grid's shape, resident on the GPU, and
 HIP events around whole batched calls on the solve stream, after warm-up
-calls.  Every candidate computes the same solve (the choices change rounding
+calls, the calls queued back to back.  Every candidate computes the same solve (the choices change rounding
 only), so tuning never changes what a caller gets beyond that.
 
 The result is a table {"JxKxL": {"cr_lanes": .., "refine_fused": ..,
@@ -51,20 +51,21 @@ def fused_threshold(times_fused: dict, times_stages: dict) -> int:
 
 
 def _time_batch(ctx, jobs, steps, reps, warmup, stream):
-    """Median milliseconds of a batched call, timed with events on the
-    stream the context issues on (the batch joins back to it)."""
+    """Median milliseconds of a batched call on the GPU: events around each
+    call on the stream the context issues on (the batch joins back to it),
+    the calls queued back to back with one wait at the end, as a stream of
+    batches runs.  (A wait after every call would add each call's launch
+    time to its span and favour the form with fewer launches.)"""
     import torch
     for _ in range(warmup):
         ctx.solve_batch(jobs, refine_steps=steps)
-    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ms = []
-    for _ in range(reps):
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for t0, t1 in ev:
         t0.record(stream)
         ctx.solve_batch(jobs, refine_steps=steps)
         t1.record(stream)
-        t1.synchronize()
-        ms.append(t0.elapsed_time(t1))
-    return float(np.median(ms))
+    ev[-1][1].synchronize()
+    return float(np.median([t0.elapsed_time(t1) for t0, t1 in ev]))
 
 
 def tune_grid(ctx: Context, g, batch_sizes=(1, 32), refine_steps: int = 2, reps: int = 5, warmup: int = 2):
@@ -134,8 +135,9 @@ def apply(pg: PreparedGrid, table: dict) -> bool:
 
 def measured_table() -> dict:
     """The table this tuner produced on an MI355X (tuned_mi355x.json beside
-    this module), refined solves of 2 steps: nine grid classes with batch
-    sizes 1, 8, 32 (1, 8 for JMAX = 512), medians of 3 calls; and the other
+    this module), refined solves of 2 steps, calls queued back to back: nine
+    grid classes with batch sizes 1, 8, 32 (1, 8 for JMAX = 512), medians of
+    3 calls; and the other
     21 classes of the heterogeneous set JMAX in {64,128,256}, KMAX in
     {32,64,128}, LMAX in {64,128,256} with batch sizes 1 and 4, medians of 5
     calls."""

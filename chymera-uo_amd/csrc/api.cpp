@@ -483,8 +483,10 @@ static int rhs_spectrum_f(chy_grid *g, const Fields &F, hipStream_t s, int maxtr
 //   then the inverse transform of X (binary64) into PHI with POT3's ghosts,
 //   and ZAXPHI.
 // The first step's R is B itself.  X lives in the grid's binary64 eigen
-// coordinate buffer (unused by this path), R and the eigen coordinates in
-// binary32 buffers.  with_stats: per step max |D| and max |X| into d.rstat.
+// coordinate buffer (unused by this path): in binary32 for two steps, whose
+// only stored iterate is the first, the binary32 sum of the FP32 matrix cores
+// (exact, half the bytes for its store and both reads); R and the eigen
+// coordinates in binary32 buffers.  with_stats: per step max |D| and max |X| into d.rstat.
 static int run_refined_fields(chy_grid *g, const std::vector<chy_grid *> &grids, const Fields &F, hipStream_t s,
                               int maxtrm, int isym, double redge, int npoint, int coef_mode, int steps,
                               bool with_stats)
@@ -515,12 +517,17 @@ static int run_refined_fields(chy_grid *g, const std::vector<chy_grid *> &grids,
         if (p) p->end(s);
         steps = 0;  // the mode work is done
     }
+    // two steps: the first iterate, a binary32 value, is kept in binary32
+    const bool x32 = steps == 2;
     for (int it = 0; it < steps && !rc; ++it) {
         const bool last = it == steps - 1;
-        const int epi = steps == 1 ? EPI_SPEC : (it == 0 ? EPI_XSET : (last ? EPI_XFIN : EPI_XADD));
+        const int epi = steps == 1 ? EPI_SPEC
+                        : it == 0  ? (x32 ? EPI_XSET32 : EPI_XSET)
+                        : last     ? (x32 ? EPI_XFIN32 : EPI_XFIN)
+                                   : EPI_XADD;
         if (it > 0) {
             if (p) p->begin(CHY_STAGE_RESID, s);
-            rc = dev_rc(launch_spec_residual_f(g, Fr, s), "spectral residual launch");
+            rc = dev_rc(launch_spec_residual_f(g, Fr, x32, s), "spectral residual launch");
             if (p) p->end(s);
         }
         if (!rc)

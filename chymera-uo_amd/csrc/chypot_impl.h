@@ -329,12 +329,15 @@ int launch_mode_solve_f(const chy_grid *g, const Fields &F, bool rhs_edges, hipS
 // [q][k][j] pitch J), eigen coordinates F.yh (binary32); the z transforms in
 // binary32 on the FP32 matrix cores, the r solves in binary32 except for
 // orders m <= m_fp64, which are solved in binary64; the correction X goes to
-// the binary64 solution spectrum F.aux per epi (EPI_XSET: X, EPI_XADD: += X,
-// EPI_XFIN: F.spec = F.aux + X).  refine_supported(g): KMAX-1 <= 127.
+// the solution spectrum F.aux per epi (EPI_XSET: X, EPI_XADD: += X,
+// EPI_XFIN: F.spec = F.aux + X; F.aux binary64, or binary32 for EPI_XSET32 and
+// EPI_XFIN32: the first iterate is the FP32 matrix cores' sum, a binary32
+// value, so a 2-step solve keeps it exactly in half the bytes).
+// refine_supported(g): KMAX-1 <= 255.
 // Ff carries the forward transform's arrays, Fi the inverse's.  stat_step >= 0:
 // the inverse also collects max |X| and max |F.aux + X| into
 // F.rstat + 2 * stat_step.
-enum { EPI_SPEC = 0, EPI_XSET = 1, EPI_XADD = 2, EPI_XFIN = 3 };
+enum { EPI_SPEC = 0, EPI_XSET = 1, EPI_XADD = 2, EPI_XFIN = 3, EPI_XSET32 = 4, EPI_XFIN32 = 5 };
 int refine_supported(const chy_grid *g);
 // The whole mode work of a 1- or 2-step refined solve in one kernel per mode
 // plane (binary32 plane in LDS): reads the binary64 right-hand side spectrum
@@ -354,8 +357,9 @@ int launch_zero_u64_f(const Fields &F, int n, hipStream_t s);
 // The boundary terms of Y on the spectrum (pot3.f:212-224), for every field.
 int launch_rhs_edges_f(const chy_grid *g, const Fields &F, hipStream_t s);
 // The mode systems' binary64 residual B - A X in the spectrum (F.spec = B,
-// F.aux = X) rounded to binary32 into F.yh ([q][k][j], pitch J).
-int launch_spec_residual_f(const chy_grid *g, const Fields &F, hipStream_t s);
+// F.aux = X, binary32 when x32 else binary64) rounded to binary32 into F.yh
+// ([q][k][j], pitch J).
+int launch_spec_residual_f(const chy_grid *g, const Fields &F, bool x32, hipStream_t s);
 
 // ZAXPHI with PHAC/PHICHK/KLOCAT (POT3's ghost cells come with the inverse FFT).
 // samples: read PHI(J=2..6,K,L) from the copy the inverse FFT just made

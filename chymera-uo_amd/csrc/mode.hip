@@ -22,6 +22,7 @@
 //   z inverse    X_q[k][j] = sum_p W_i[k][p] z[q][p][j], written back into the
 //                spectrum (GEMM per mode)
 #include <algorithm>
+#include <type_traits>
 
 #include "chypot_impl.h"
 
@@ -197,6 +198,8 @@ template <> struct Mfma<float> {
 //   EPI_XSET  into the binary64 solution spectrum F.aux ([q][p][j] layout)
 //   EPI_XADD  added to F.aux
 //   EPI_XFIN  F.aux + X into the binary64 spectrum (the refinement's last step)
+//   EPI_XSET32, EPI_XFIN32  the same with F.aux in binary32 (AT = float only:
+//             X is then the binary32 sum of the products, stored exactly)
 // AT: arithmetic of the products (binary64, or binary32 on the FP32 matrix
 // cores for the refinement's correction solves); TI, TO: element types of the
 // product's input and output (the forward transform reads the spectrum and
@@ -206,6 +209,11 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
                                                                          int ntiles)
 {
     typedef Mfma<AT> MM;
+    constexpr bool X32 = EPI == EPI_XSET32 || EPI == EPI_XFIN32;
+    static_assert(!X32 || sizeof(AT) == 4, "a binary32 solution spectrum holds binary32 sums only");
+    typedef typename std::conditional<X32, float, double>::type XT;
+    constexpr bool XSET = EPI == EPI_XSET || EPI == EPI_XSET32;
+    constexpr bool XFIN = EPI == EPI_XFIN || EPI == EPI_XFIN32;
     // tile t: field t / tpf, then mode and column block within the field
     const int tpf = col_tiles * a.L;
     constexpr int NT = PPT * 4;            // threads (PPT/16 waves)
@@ -268,11 +276,11 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
         if (next < ntiles) fetch(next);
         // the solution spectrum this tile's results are added to, loaded
         // before the products so that the epilogue does not wait for it
-        double xold[U][4];
-        if constexpr (!FWD && (EPI == EPI_XADD || EPI == EPI_XFIN)) {
+        XT xold[U][4];
+        if constexpr (!FWD && (EPI == EPI_XADD || XFIN)) {
             const int f0 = tile / tpf, t0 = tile % tpf;
             const int q0 = t0 / col_tiles, c00 = (t0 % col_tiles) * PCT;
-            const double *xf0 = static_cast<const double *>(F.aux[f0]);
+            const XT *xf0 = static_cast<const XT *>(F.aux[f0]);
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -297,7 +305,7 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
         const int q = tt / col_tiles, c0 = (tt % col_tiles) * PCT;
         TO *const yhf = static_cast<TO *>(F.yh[f]);
         TO *const specf = static_cast<TO *>(F.spec[f]);
-        double *const xf = static_cast<double *>(F.aux[f]);
+        XT *const xf = static_cast<XT *>(F.aux[f]);
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -312,13 +320,13 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
                     const double x = (double)acc[u][v];
                     double xn = x;
                     if constexpr (EPI == EPI_SPEC) specf[((size_t)q * a.K + r) * a.J + j] = (TO)acc[u][v];
-                    if constexpr (EPI == EPI_XSET) xf[xi] = x;
+                    if constexpr (XSET) xf[xi] = (XT)x;
                     if constexpr (EPI == EPI_XADD) {
-                        xn = xold[u][v] + x;
+                        xn = (double)xold[u][v] + x;
                         xf[xi] = xn;
                     }
-                    if constexpr (EPI == EPI_XFIN) {
-                        xn = xold[u][v] + x;
+                    if constexpr (XFIN) {
+                        xn = (double)xold[u][v] + x;
                         static_cast<double *>(F.spec[f])[((size_t)q * a.K + r) * a.J + j] = xn;
                     }
                     if (!FWD && a.stat_step >= 0) {
@@ -658,13 +666,16 @@ ModeArgs make_args(const chy_grid *g)
 template <bool FWD, typename TI, typename TO, typename AT = double, int EPI = EPI_SPEC>
 void launch_zgemm(const chy_grid *g, const ModeArgs &a, const Fields &F, hipStream_t s)
 {
-    // 64-column tiles, one persistent block per CU (128 KB / 64 KB of LDS)
-    constexpr int PCT = 64;
+    // 64-column tiles, one persistent block per CU (128 KB / 64 KB of LDS);
+    // the last binary32 inverse (binary32 X read, the binary64 spectrum
+    // written) in 32-column tiles, whose epilogue fits two blocks per CU
+    // (993 against 1147 us per 32-field launch at 256x128x256)
+    constexpr int PCT = EPI == EPI_XFIN32 ? 32 : 64;
     if (g->PP == 128 || g->PP == 64) {
-        // binary32 forward products: two blocks per CU (half the LDS of a
-        // binary64 block; the inverse's epilogue needs more than the 128
-        // registers that would leave)
-        constexpr int MB = (sizeof(AT) == 4 && FWD) ? 2 : 1;
+        // binary32 forward products and binary32-X inverses: two blocks per
+        // CU (half the LDS of a binary64 block; the binary64-X inverses'
+        // epilogues need more than the 128 registers that would leave)
+        constexpr int MB = (sizeof(AT) == 4 && (FWD || EPI == EPI_XSET32 || EPI == EPI_XFIN32)) ? 2 : 1;
         const int col_tiles = ((FWD ? g->JP : g->M) + PCT - 1) / PCT;
         const int ntiles = col_tiles * g->L * F.nf;
         const int blocks = std::min(ntiles, MB * g->ncu);
@@ -778,6 +789,8 @@ int launch_mode_solve32_f(const chy_grid *g, const Fields &Ff, const Fields &Fi,
     case EPI_XSET: launch_zgemm<false, float, double, float, EPI_XSET>(g, a, Fi, s); break;
     case EPI_XADD: launch_zgemm<false, float, double, float, EPI_XADD>(g, a, Fi, s); break;
     case EPI_XFIN: launch_zgemm<false, float, double, float, EPI_XFIN>(g, a, Fi, s); break;
+    case EPI_XSET32: launch_zgemm<false, float, double, float, EPI_XSET32>(g, a, Fi, s); break;
+    case EPI_XFIN32: launch_zgemm<false, float, double, float, EPI_XFIN32>(g, a, Fi, s); break;
     default: launch_zgemm<false, float, double, float, EPI_SPEC>(g, a, Fi, s); break;
     }
     if (prof) prof->end(s);
@@ -807,7 +820,8 @@ int launch_rhs_edges_f(const chy_grid *g, const Fields &F, hipStream_t s)
 // Residual of the mode systems in the azimuthal spectrum, binary64:
 //   R = B - (AN X(k-1) + AM X(j-1) + (BN + BM_m) X + CM X(j+1) + CN X(k+1))
 // with B the right-hand side spectrum (F.spec, boundary terms applied) and X
-// the binary64 solution spectrum (F.aux, [q][p][j]), rounded to binary32 into
+// the solution spectrum (F.aux, [q][p][j], binary64, or binary32 when XT is:
+// the first iterate of a 2-step solve), rounded to binary32 into
 // F.yh, a [q][k][j] array of pitch J (the next correction's right-hand side).
 // The azimuthal DFT diagonalises the 7-point operator exactly (its L
 // difference becomes 2 (cos m DTHETA - 1) C in BM_m, the axis coupling the
@@ -818,6 +832,7 @@ int launch_rhs_edges_f(const chy_grid *g, const Fields &F, hipStream_t s)
 // kept a separate stream open in each of thousands of planes); every load of
 // the block's rows is issued before the first is used.
 constexpr int RROWS = 8;
+template <typename XT>
 __global__ __launch_bounds__(256) void k_spec_residual(ModeArgs a, const Fields F, const double *bn)
 {
     const int q = blockIdx.y, fi = blockIdx.z;
@@ -832,24 +847,35 @@ __global__ __launch_bounds__(256) void k_spec_residual(ModeArgs a, const Fields 
     const int m = q <= a.N ? q : q - a.N;
     const int NZ = a.NZ;
     const double *B = static_cast<const double *>(F.spec[fi]) + (size_t)q * a.K * a.J + jc;
-    const double *X = static_cast<const double *>(F.aux[fi]) + (size_t)q * a.PP * a.JP + jc;
+    const XT *X = static_cast<const XT *>(F.aux[fi]) + (size_t)q * a.PP * a.JP + jc;
     float *Rf = static_cast<float *>(F.yh[fi]) + (size_t)q * a.K * a.J + jc;
     const double amj = a.am[jc], cmj = a.cm[jc], bmj = a.bm[(size_t)m * a.JP + jc];  // AM(1) = CM(M) = 0
-    // the wavefront's edge lanes take their outer neighbour from memory
+    // the wavefront's edge lanes take their outer neighbour from memory (lane
+    // 0 the left one, lane 63 the right one, the others their own column)
     const bool jl = j > 0, jr = j + 1 < a.M;
-    // X rows k0-1 .. k0+RROWS, B rows k0 .. k0+RROWS-1, the edge lanes' outer neighbours
-    double xs[RROWS + 2], bs[RROWS], el[RROWS], er[RROWS];
+    const int eoff = (lane == 0 && jl) ? -1 : ((lane == 63 && jr) ? 1 : 0);
+    // X rows k0-1 .. k0+RROWS, B rows k0 .. k0+RROWS-1, the edge neighbours:
+    // every load at a clamped row, issued before any is used (a load under a
+    // branch waits inside it when its value is converted there), then the
+    // rows outside 0..NZ-1 selected away
+    XT xv[RROWS + 2], ev[RROWS];
+    double bs[RROWS];
 #pragma unroll
     for (int i = 0; i < RROWS + 2; ++i) {
-        const int k = k0 - 1 + i;
-        xs[i] = (k >= 0 && k < NZ) ? X[(size_t)k * a.JP] : 0.0;
+        const int k = k0 - 1 + i, kc = k < 0 ? 0 : (k < NZ ? k : NZ - 1);
+        xv[i] = X[(size_t)kc * a.JP];
     }
 #pragma unroll
     for (int i = 0; i < RROWS; ++i) {
-        const int k = k0 + i;
-        bs[i] = k < NZ ? B[(size_t)k * a.J] : 0.0;
-        el[i] = (lane == 0 && jl && k < NZ) ? X[(size_t)k * a.JP - 1] : 0.0;
-        er[i] = (lane == 63 && jr && k < NZ) ? X[(size_t)k * a.JP + 1] : 0.0;
+        const int k = k0 + i, kc = k < NZ ? k : NZ - 1;
+        bs[i] = B[(size_t)kc * a.J];
+        ev[i] = X[(size_t)kc * a.JP + eoff];
+    }
+    double xs[RROWS + 2];
+#pragma unroll
+    for (int i = 0; i < RROWS + 2; ++i) {
+        const int k = k0 - 1 + i;
+        xs[i] = (k >= 0 && k < NZ) ? (double)xv[i] : 0.0;
     }
 #pragma unroll
     for (int i = 0; i < RROWS; ++i) {
@@ -857,8 +883,8 @@ __global__ __launch_bounds__(256) void k_spec_residual(ModeArgs a, const Fields 
         if (k >= NZ) break;  // the same k for the whole block
         const double xm = xs[i], xc = xs[i + 1], xp = xs[i + 2];
         double xl = __shfl_up(xc, 1, 64), xrt = __shfl_down(xc, 1, 64);
-        if (lane == 0) xl = el[i];
-        if (lane == 63) xrt = er[i];
+        if (lane == 0) xl = jl ? (double)ev[i] : 0.0;
+        if (lane == 63) xrt = jr ? (double)ev[i] : 0.0;
         const double ax = a.an_dev[k] * xm + (bn[k] + bmj) * xc + a.cn_dev[k] * xp + amj * xl + cmj * xrt;
         if (live) Rf[(size_t)k * a.J] = (float)(bs[i] - ax);
     }
@@ -1341,13 +1367,16 @@ void launch_refine_plane(const chy_grid *g, const ModeArgs &a, const Fields &F, 
 
 }  // namespace
 
-int launch_spec_residual_f(const chy_grid *g, const Fields &F, hipStream_t s)
+int launch_spec_residual_f(const chy_grid *g, const Fields &F, bool x32, hipStream_t s)
 {
     ModeArgs a = make_args(g);
     a.an_dev = g->d.rz_an;
     a.cn_dev = g->d.rz_cn;
     const dim3 grid((unsigned)(((g->NZ + RROWS - 1) / RROWS) * ((g->M + 255) / 256)), g->L, F.nf);
-    hipLaunchKernelGGL(k_spec_residual, grid, dim3(256), 0, s, a, F, g->d.rz_bn);
+    if (x32)
+        hipLaunchKernelGGL(k_spec_residual<float>, grid, dim3(256), 0, s, a, F, g->d.rz_bn);
+    else
+        hipLaunchKernelGGL(k_spec_residual<double>, grid, dim3(256), 0, s, a, F, g->d.rz_bn);
     return (int)hipGetLastError();
 }
 
