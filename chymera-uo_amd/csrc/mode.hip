@@ -193,6 +193,12 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+// 4 consecutive floats as one 16-byte store (p 16-byte aligned)
+__device__ __forceinline__ void st4(float *p, const float (&x)[4])
+{
+    *reinterpret_cast<float4 *>(p) = make_float4(x[0], x[1], x[2], x[3]);
+}
+
 // Where the z inverse puts its result X (rows k < NZ, columns j < M):
 //   EPI_SPEC  into the spectrum (type TO)
 //   EPI_XSET  into the binary64 solution spectrum F.aux ([q][p][j] layout)
@@ -226,6 +232,16 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
     const int nred = a.NZ;
     const int nrow = FWD ? a.PP : a.NZ;
     const double *W = FWD ? a.wf : a.wi;
+    // binary32 products with binary32 results run transposed (the LDS tile
+    // as the first operand of V_MFMA_F32_16X16X4F32): a lane then holds 4
+    // consecutive columns j of one row r = 16w + lr and stores them as one
+    // 16-byte vector (a store instruction covers 16 rows x 64 bytes).  Forward
+    // products 815 -> 756 and 741 -> 707 us, the X store 776 -> 764 us per
+    // 32-field launch at 256x128x256.  Binary64 results stay in the other
+    // layout, where 16 lanes write one row's 128 contiguous bytes: as two
+    // 16-byte halves per lane they left holes in every store (999 -> 1228 us).
+    // (The binary64 product's own layout has no run of columns in a lane.)
+    constexpr bool TR = sizeof(AT) == 4 && ((FWD && sizeof(TO) == 4) || EPI == EPI_XSET32);
 
     int tile = blockIdx.x;
     if (tile >= ntiles) return;  // the whole block leaves together
@@ -298,7 +314,10 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const AT b = Bs[buf][4 * ks + lk][16 * u + lr];
-                acc[u] = MM::mac(ar[ks], b, acc[u]);
+                if constexpr (TR)
+                    acc[u] = MM::mac(b, ar[ks], acc[u]);
+                else
+                    acc[u] = MM::mac(ar[ks], b, acc[u]);
             }
         }
         const int f = tile / tpf, tt = tile % tpf;
@@ -306,6 +325,44 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
         TO *const yhf = static_cast<TO *>(F.yh[f]);
         TO *const specf = static_cast<TO *>(F.spec[f]);
         XT *const xf = static_cast<XT *>(F.aux[f]);
+        if constexpr (TR) {
+            // rows r = 16w + lr, columns j0..j0+3; every vector lies inside
+            // its row (JP is a multiple of the tile width), the stores of the
+            // X rows stop at M
+            const int r = 16 * w + lr;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j0 = c0 + 16 * u + 4 * lk;
+                float o[4];
+                if constexpr (FWD) {
+                    if (r < a.PP) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) o[v] = (r < a.NZ && j0 + v < a.M) ? acc[u][v] : 0.0f;
+                        st4(reinterpret_cast<float *>(yhf) + ((size_t)q * a.PP + r) * a.JP + j0, o);
+                    }
+                } else if (r < a.NZ) {  // EPI_XSET32
+                    const int n = a.M - j0;  // columns of the 4 inside the system
+                    float *const p = reinterpret_cast<float *>(xf) + ((size_t)q * a.PP + r) * a.JP + j0;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) o[v] = acc[u][v];
+                    if (n >= 4) {
+                        st4(p, o);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            if (v < n) p[v] = o[v];
+                    }
+                    if (a.stat_step >= 0) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            if (v < n) {
+                                smax[0] = fmax(smax[0], fabs((double)o[v]));
+                                smax[1] = smax[0];
+                            }
+                    }
+                }
+            }
+        } else {
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -335,6 +392,7 @@ __global__ __launch_bounds__(PPT * 4, MINB * PPT / 64) void k_zgemm_reg(ModeArgs
                     }
                 }
             }
+        }
         // the refinement's step statistics: max |correction| and max |solution|
         // of the field, as the bit patterns of non-negative doubles
         if (!FWD && a.stat_step >= 0) {
