@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "chypot_impl.h"
+#include "lanes.h"
 
 namespace chy {
 
@@ -61,17 +62,22 @@ __device__ __forceinline__ Red identity_unless(bool keep, const Red &r)
 }
 
 // neighbour s lanes below / above within a group of G lanes; identity rows
-// outside the group
+// outside the group (distance 1 by DPP moves)
 template <int G>
 __device__ __forceinline__ Red red_up(const Red &r, int s, int gl)
 {
     Red o;
-    o.al = __shfl_up(r.al, s, G);
-    o.be = __shfl_up(r.be, s, G);
-    o.ga = __shfl_up(r.ga, s, G);
-    o.d0 = __shfl_up(r.d0, s, G);
-    o.d1 = __shfl_up(r.d1, s, G);
-    o.inv = __shfl_up(r.inv, s, G);
+    if (s == 1) {
+        o = Red{lane_prev(r.al), lane_prev(r.be), lane_prev(r.ga), lane_prev(r.d0), lane_prev(r.d1),
+                lane_prev(r.inv)};
+    } else {
+        o.al = __shfl_up(r.al, s, G);
+        o.be = __shfl_up(r.be, s, G);
+        o.ga = __shfl_up(r.ga, s, G);
+        o.d0 = __shfl_up(r.d0, s, G);
+        o.d1 = __shfl_up(r.d1, s, G);
+        o.inv = __shfl_up(r.inv, s, G);
+    }
     return identity_unless(gl >= s, o);
 }
 
@@ -79,12 +85,17 @@ template <int G>
 __device__ __forceinline__ Red red_down(const Red &r, int s, int gl)
 {
     Red o;
-    o.al = __shfl_down(r.al, s, G);
-    o.be = __shfl_down(r.be, s, G);
-    o.ga = __shfl_down(r.ga, s, G);
-    o.d0 = __shfl_down(r.d0, s, G);
-    o.d1 = __shfl_down(r.d1, s, G);
-    o.inv = __shfl_down(r.inv, s, G);
+    if (s == 1) {
+        o = Red{lane_next(r.al), lane_next(r.be), lane_next(r.ga), lane_next(r.d0), lane_next(r.d1),
+                lane_next(r.inv)};
+    } else {
+        o.al = __shfl_down(r.al, s, G);
+        o.be = __shfl_down(r.be, s, G);
+        o.ga = __shfl_down(r.ga, s, G);
+        o.d0 = __shfl_down(r.d0, s, G);
+        o.d1 = __shfl_down(r.d1, s, G);
+        o.inv = __shfl_down(r.inv, s, G);
+    }
     return identity_unless(gl + s < G, o);
 }
 
@@ -129,8 +140,8 @@ __device__ __forceinline__ void tri_solve(const double (&ca)[R], const double (&
             P[i] = wp[i] - g[i] * P[i + 1];
             Q[i] = wq[i] - g[i] * Q[i + 1];
         }
-        const double nD0 = __shfl_down(D0[0], 1, G), nD1 = __shfl_down(D1[0], 1, G);
-        const double nP = __shfl_down(P[0], 1, G), nQ = __shfl_down(Q[0], 1, G);
+        const double nD0 = lane_next(D0[0]), nD1 = lane_next(D1[0]);
+        const double nP = lane_next(P[0]), nQ = lane_next(Q[0]);
         const bool last = gl == G - 1;
         const double al = ca[R - 1], be = cb[R - 1], c = last ? 0.0 : cc[R - 1];
         red.al = al * P[R - 2];
@@ -162,7 +173,7 @@ __device__ __forceinline__ void tri_solve(const double (&ca)[R], const double (&
     x0[R - 1] = xr0;
     x1[R - 1] = xr1;
     if constexpr (R > 1) {
-        double xl0 = __shfl_up(xr0, 1, G), xl1 = __shfl_up(xr1, 1, G);
+        double xl0 = lane_prev(xr0), xl1 = lane_prev(xr1);
         if (gl == 0) { xl0 = 0.0; xl1 = 0.0; }
 #pragma unroll
         for (int i = 0; i < R - 1; ++i) {

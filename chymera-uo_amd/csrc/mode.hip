@@ -25,6 +25,7 @@
 #include <type_traits>
 
 #include "chypot_impl.h"
+#include "lanes.h"
 
 namespace chy {
 namespace {
@@ -438,15 +439,21 @@ __device__ __forceinline__ Red<AT> identity_unless(bool keep, const Red<AT> &r)
                    keep ? r.d1 : (AT)0};
 }
 
+// the row s lanes below / above (distance 1 by DPP moves); identity rows
+// past the wavefront's ends
 template <typename AT>
 __device__ __forceinline__ Red<AT> shfl_up_red(const Red<AT> &r, int s, int lane)
 {
     Red<AT> o;
-    o.al = __shfl_up(r.al, s, 64);
-    o.be = __shfl_up(r.be, s, 64);
-    o.ga = __shfl_up(r.ga, s, 64);
-    o.d0 = __shfl_up(r.d0, s, 64);
-    o.d1 = __shfl_up(r.d1, s, 64);
+    if (s == 1) {
+        o = Red<AT>{lane_prev(r.al), lane_prev(r.be), lane_prev(r.ga), lane_prev(r.d0), lane_prev(r.d1)};
+    } else {
+        o.al = __shfl_up(r.al, s, 64);
+        o.be = __shfl_up(r.be, s, 64);
+        o.ga = __shfl_up(r.ga, s, 64);
+        o.d0 = __shfl_up(r.d0, s, 64);
+        o.d1 = __shfl_up(r.d1, s, 64);
+    }
     return identity_unless(lane >= s, o);
 }
 
@@ -454,11 +461,15 @@ template <typename AT>
 __device__ __forceinline__ Red<AT> shfl_down_red(const Red<AT> &r, int s, int lane)
 {
     Red<AT> o;
-    o.al = __shfl_down(r.al, s, 64);
-    o.be = __shfl_down(r.be, s, 64);
-    o.ga = __shfl_down(r.ga, s, 64);
-    o.d0 = __shfl_down(r.d0, s, 64);
-    o.d1 = __shfl_down(r.d1, s, 64);
+    if (s == 1) {
+        o = Red<AT>{lane_next(r.al), lane_next(r.be), lane_next(r.ga), lane_next(r.d0), lane_next(r.d1)};
+    } else {
+        o.al = __shfl_down(r.al, s, 64);
+        o.be = __shfl_down(r.be, s, 64);
+        o.ga = __shfl_down(r.ga, s, 64);
+        o.d0 = __shfl_down(r.d0, s, 64);
+        o.d1 = __shfl_down(r.d1, s, 64);
+    }
     return identity_unless(lane + s < 64, o);
 }
 
@@ -553,8 +564,8 @@ __device__ __forceinline__ void rsolve_rows(const ModeArgs &a, int m, int p, int
             Q[i] = wq[i] - g[i] * Q[i + 1];
         }
         // first-row data of the next lane
-        const AT nD0 = __shfl_down(D0[0], 1, 64), nD1 = __shfl_down(D1[0], 1, 64);
-        const AT nP = __shfl_down(P[0], 1, 64), nQ = __shfl_down(Q[0], 1, 64);
+        const AT nD0 = lane_next(D0[0]), nD1 = lane_next(D1[0]);
+        const AT nP = lane_next(P[0]), nQ = lane_next(Q[0]);
         const AT al = ca[R - 1], be = cb[R - 1], c = lane < 63 ? cc[R - 1] : (AT)0;
         const AT nPv = lane < 63 ? nP : (AT)0, nQv = lane < 63 ? nQ : (AT)0;
         const AT nD0v = lane < 63 ? nD0 : (AT)0, nD1v = lane < 63 ? nD1 : (AT)0;
@@ -586,7 +597,7 @@ __device__ __forceinline__ void rsolve_rows(const ModeArgs &a, int m, int p, int
     x0[R - 1] = xr0;
     x1[R - 1] = xr1;
     if constexpr (R > 1) {
-        AT xl0 = __shfl_up(xr0, 1, 64), xl1 = __shfl_up(xr1, 1, 64);
+        AT xl0 = lane_prev(xr0), xl1 = lane_prev(xr1);
         if (lane == 0) { xl0 = (AT)0; xl1 = (AT)0; }
 #pragma unroll
         for (int i = 0; i < R - 1; ++i) {
@@ -942,7 +953,7 @@ __global__ __launch_bounds__(256) void k_spec_residual(ModeArgs a, const Fields 
         const int k = k0 + i;
         if (k >= NZ) break;  // the same k for the whole block
         const double xm = xs[i], xc = xs[i + 1], xp = xs[i + 2];
-        double xl = __shfl_up(xc, 1, 64), xrt = __shfl_down(xc, 1, 64);
+        double xl = lane_prev(xc), xrt = lane_next(xc);
         if (lane == 0) xl = jl ? (double)ev[i] : 0.0;
         if (lane == 63) xrt = jr ? (double)ev[i] : 0.0;
         const double ax = a.an_dev[k] * xm + (bn[k] + bmj) * xc + a.cn_dev[k] * xp + amj * xl + cmj * xrt;
@@ -1013,7 +1024,7 @@ __device__ __forceinline__ void rsolve1(const AT (&ca)[R], const AT (&cb)[R], co
             P[i] = wp[i] - g[i] * P[i + 1];
             Q[i] = wq[i] - g[i] * Q[i + 1];
         }
-        const AT nD = __shfl_down(D[0], 1, 64), nP = __shfl_down(P[0], 1, 64), nQ = __shfl_down(Q[0], 1, 64);
+        const AT nD = lane_next(D[0]), nP = lane_next(P[0]), nQ = lane_next(Q[0]);
         const bool last = lane == 63;
         const AT c = last ? (AT)0 : cc[R - 1];
         al = ca[R - 1] * P[R - 2];
@@ -1023,10 +1034,16 @@ __device__ __forceinline__ void rsolve1(const AT (&ca)[R], const AT (&cb)[R], co
     }
 #pragma unroll
     for (int s = 1; s < 64; s <<= 1) {
-        AT lal = __shfl_up(al, s, 64), lbe = __shfl_up(be, s, 64), lga = __shfl_up(ga, s, 64);
-        AT ld = __shfl_up(dd, s, 64);
-        AT hal = __shfl_down(al, s, 64), hbe = __shfl_down(be, s, 64), hga = __shfl_down(ga, s, 64);
-        AT hd = __shfl_down(dd, s, 64);
+        AT lal, lbe, lga, ld, hal, hbe, hga, hd;
+        if (s == 1) {
+            lal = lane_prev(al); lbe = lane_prev(be); lga = lane_prev(ga); ld = lane_prev(dd);
+            hal = lane_next(al); hbe = lane_next(be); hga = lane_next(ga); hd = lane_next(dd);
+        } else {
+            lal = __shfl_up(al, s, 64); lbe = __shfl_up(be, s, 64); lga = __shfl_up(ga, s, 64);
+            ld = __shfl_up(dd, s, 64);
+            hal = __shfl_down(al, s, 64); hbe = __shfl_down(be, s, 64); hga = __shfl_down(ga, s, 64);
+            hd = __shfl_down(dd, s, 64);
+        }
         // identity rows past the ends, by selects (see identity_unless)
         const bool kl = lane >= s, kh = lane + s < 64;
         lal = kl ? lal : (AT)0; lbe = kl ? lbe : (AT)1; lga = kl ? lga : (AT)0; ld = kl ? ld : (AT)0;
@@ -1040,7 +1057,7 @@ __device__ __forceinline__ void rsolve1(const AT (&ca)[R], const AT (&cb)[R], co
     const AT xr = qdiv(dd, be);
     x[R - 1] = xr;
     if constexpr (R > 1) {
-        AT xl = __shfl_up(xr, 1, 64);
+        AT xl = lane_prev(xr);
         if (lane == 0) xl = (AT)0;
 #pragma unroll
         for (int i = 0; i < R - 1; ++i) x[i] = D[i] + P[i] * xl + Q[i] * xr;
