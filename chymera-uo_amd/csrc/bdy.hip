@@ -49,7 +49,7 @@ struct BdyArgs {
     double cnmax, cmmax;          // boundary coefficients of the mode RHS (pot3.f:184-191)
 };
 
-// field f's arrays (blockIdx.y)
+// field f's arrays (f: the block's field index)
 __device__ __forceinline__ void bind(BdyArgs &a, const Fields &F, int f)
 {
     a.phi = F.phi[f];
@@ -62,13 +62,16 @@ __device__ __forceinline__ void bind(BdyArgs &a, const Fields &F, int f)
 // chunk[0][2t+s][c]: sum over chunk c's rings of TERM_t * moment_s (inside
 // form), chunk[1][2t+s][c]: the same with TERM_t r^-(2l+1) (outside form);
 // s=0 cosine (SUMASS for m=0), s=1 sine.
+// blockIdx.x is the field and blockIdx.y the chunk, so the fields' blocks of
+// one chunk run together and share the chunk's rows of the grid's tables
+// (TERM, r) in L2 instead of each field streaming the whole tables again.
 __global__ __launch_bounds__(256) void k_bdy_chunks(BdyArgs a, const Fields F)
 {
-    bind(a, F, blockIdx.y);
+    bind(a, F, blockIdx.x);
     constexpr int IT = (kChunkRings + CHUNK_GROUPS - 1) / CHUNK_GROUPS;  // rings per thread
     __shared__ double smom[kChunkRings][kMoments + 1];
     __shared__ double part[CHUNK_GROUPS][kMaxTerms][4];
-    const int c = blockIdx.x;
+    const int c = blockIdx.y;
     const int tid = threadIdx.x;
     const int p0 = a.chunk_start[c], p1 = a.chunk_start[c + 1];
     const int t = tid % kMaxTerms, grp = tid / kMaxTerms;
@@ -443,7 +446,7 @@ int launch_bdygen_from_moments_f(const chy_grid *g, const Fields &F, int maxtrm,
     a.redge = redge; a.grav = g->grav; a.pi = g->pi;
     a.cnmax = g->cnmax; a.cmmax = g->cmmax;
     const int nf = F.nf;
-    hipLaunchKernelGGL(k_bdy_chunks, dim3(g->nchunk, nf), dim3(256), 0, s, a, F);
+    hipLaunchKernelGGL(k_bdy_chunks, dim3(nf, g->nchunk), dim3(256), 0, s, a, F);
     hipLaunchKernelGGL(k_bdy_scan, dim3(2 * NCOL, nf), dim3(256), 0, s, a, F);
     if (spectral) {
         hipLaunchKernelGGL(k_bdy_spec, dim3(g->jkmax, nf), dim3(256), 0, s, a, F, g->d.twh, coef_mode, 1);
