@@ -28,7 +28,9 @@ kernel for all of them):
     around each stage on the stream the batched launches run on): algorithmic
     bytes per field (each input read and each output written once: forward transform 134.2 MB,
     cyclic reduction 133.4 MB, inverse 135.8 MB at 256x128x256) over the
-    stage time, against 8 TB/s; the z transforms also against the matrix-core
+    stage time, against 8 TB/s and (`frac_of_copy`) against a 2 GiB device
+    copy timed in the same run (`roofline.hbm_copy_GBs`, read + write, beyond
+    the Infinity Cache); the z transforms also against the matrix-core
     peak, the cyclic reduction against the FP64 vector peak; where profiles/
     holds a rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the workload, the HBM
     bytes counted per launch beside the algorithmic ones; `roofline` is the
@@ -153,11 +155,35 @@ def hbm_traffic(stage, launches_per_step, batch, workload):
     return tot / launches_per_step if seen else None
 
 
-def rooflines(stage_times, work, nsolve, fields_per_launch, steps, workload):
+def copy_rate(stream, mib=2048, reps=10):
+    """GB/s (read + write) of a device-to-device copy of `mib` MiB, beyond the
+    256 MB Infinity Cache: the practical rate of a kernel that streams as
+    many bytes in as out, measured in the same run as the stages."""
+    import torch
+    src = torch.ones(mib << 17, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    with torch.cuda.stream(stream):
+        for _ in range(2):
+            dst.copy_(src)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(reps):
+            dst.copy_(src)
+        t1.record(stream)
+    t1.synchronize()
+    gbs = 2.0 * src.numel() * 8 * reps / (t0.elapsed_time(t1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
+def rooflines(stage_times, work, nsolve, fields_per_launch, steps, workload, copy_gbs=None):
     """Per stage of a timed region of nsolve solves over `steps` batch steps:
-    total ms, launches, the algorithmic rate against the HBM peak (and the
-    compute peak the stage's flops are priced against), and the HBM bytes
-    rocprofv3 counted per launch (profiles/) beside the algorithmic ones."""
+    total ms, launches, the algorithmic rate against the HBM peak (and
+    against a large device copy's rate measured in the same run, copy_gbs,
+    and the compute peak the stage's flops are priced against), and the HBM
+    bytes rocprofv3 counted per launch (profiles/) beside the algorithmic
+    ones."""
     out = {}
     for st, (ms, n) in stage_times.items():
         if not n or st not in work:
@@ -168,6 +194,8 @@ def rooflines(stage_times, work, nsolve, fields_per_launch, steps, workload):
         if w["bytes"]:
             gbs = w["bytes"] * nsolve / (ms * 1e-3) / 1e9
             e.update(bytes_per_field=w["bytes"], achieved_GBs=round(gbs, 1), frac_hbm=round(gbs / HBM_PEAK_GBS, 4))
+            if copy_gbs:
+                e["frac_of_copy"] = round(gbs / copy_gbs, 4)
             tr = hbm_traffic(st, n / steps, fields_per_launch, workload)
             if tr is not None:
                 alg = w["bytes"] * nsolve / n
@@ -470,6 +498,7 @@ def main():
         if args.refine_fused >= 0:
             j["grid"].refine_fused = args.refine_fused
 
+    copy_gbs = copy_rate(stream)
     if args.copy_mib > 0:
         src = torch.ones(args.copy_mib << 17, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -531,7 +560,7 @@ def main():
         if steps and st.get("planes", (0, 0))[1]:
             kind = "refined_fused"
         work = stage_work(g, kind, steps)
-        res["rooflines"] = rooflines(st, work, B * args.steps, B, args.steps, f"potsolve_{g.name}_{kind}")
+        res["rooflines"] = rooflines(st, work, B * args.steps, B, args.steps, f"potsolve_{g.name}_{kind}", copy_gbs)
         if n_lat > 0:
             one = single(steps)
             for _ in range(3):
@@ -609,7 +638,8 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": d.get("achieved_GBs"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": d.get("frac_hbm")}
-    roof.update({"traffic": d.get("traffic_bytes_per_launch"), "algorithmic_bytes_per_launch":
+    roof.update({"hbm_copy_GBs": round(copy_gbs, 1), "hbm_frac_of_copy": d.get("frac_of_copy"),
+                 "traffic": d.get("traffic_bytes_per_launch"), "algorithmic_bytes_per_launch":
                  None if not d.get("bytes_per_field") else round(d["bytes_per_field"] * B * args.steps / d["launches"]),
                  "kernel": dom, "avg_ms": d["ms_per_launch"], "launches": d["launches"], "fields_per_launch": B})
     out = {
