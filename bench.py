@@ -448,6 +448,8 @@ def main():
     ap.add_argument("--cr-lanes", type=int, default=0, help="set the cyclic reduction's lanes per r system (64, 32)")
     ap.add_argument("--refine-fused", type=int, default=-1,
                     help="set the largest fields-per-launch of the one-kernel-per-plane refined solve")
+    ap.add_argument("--coef-full-steps", type=int, default=5,
+                    help="batch steps of the refined solve writing COEF for every K (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="before the solves, copy a buffer of this many MiB device to device "
                          "three times (a kernel of known traffic for scaling HBM counters)")
@@ -602,6 +604,23 @@ def main():
                 ctx.sync()
                 per_step.append(float(np.abs(chypot.to_host(tmp) - plain).max() / pscale))
             refined["error_vs_plain_fp64_per_step"] = per_step
+    if refined is not None and args.coef_full_steps > 0:
+        # the same batched refined solve writing COEF for every K, as the
+        # reference's POT3 does (the timed region above writes the K=2 plane),
+        # beside the K=2 form timed the same way (HIP graph replays)
+        cf = {}
+        for mode, key in ((chypot.COEF_K2, "k2_plane"), (chypot.COEF_FULL, "every_k")):
+            for j in jobs:
+                j["coef_mode"] = mode
+            fn = batch_step(args.refine_steps)
+            for _ in range(2):
+                fn()
+            el, lat = timed(fn, args.coef_full_steps)
+            cf[key] = {"solves_per_s": B * args.coef_full_steps / el, "batch_step_p50_ms": float(np.percentile(lat, 50))}
+        for j in jobs:
+            j["coef_mode"] = chypot.COEF_K2
+        cf["steps"] = args.coef_full_steps
+        refined["coef"] = cf
     if "refined_fused" in paths:
         # the same refined solve with all mode work of a plane in one kernel
         # (binary32 plane in LDS, one HBM round trip): the comparison row
